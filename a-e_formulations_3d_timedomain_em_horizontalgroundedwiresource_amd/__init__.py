@@ -1,0 +1,217 @@
+"""MI355X-native implicit 3D time-domain EM step (drop-in for ``CalcNonstat3DLine``).
+
+Python bindings (ctypes) over the C ABI of ``libemtd.so`` (``include/emtd.h``). The library
+holds all compute: HIP kernels for gfx950 plus C++ host code. This module only marshals
+arrays; it fails loudly when the native library is missing.
+
+Mirrors of the reference interfaces:
+
+* :func:`portrait`      — ``T_Portrait::Gen_Portrait`` (``A2/T_Portrait.cpp:45-122``)
+* :func:`assemble`      — ``T_Global_SLAE_Vec::Assembling`` (``A2/T_Global_SLAE.cpp:76-215``)
+* :class:`Solver`       — ``pardiso_solver`` (``A2/pardiso.h:25-53``)
+* :class:`Calc`         — ``CalcSP`` / ``Time_approx_for_vfem`` (``A2/TaskSP.cpp:445-711``)
+* :func:`calc_nonstat_3d_line` — the whole module as ``CalcNonstat3DLine.exe`` runs it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemtd.so")
+
+FORM_A, FORM_E = 0, 1
+FP64, FP32_IR = 0, 1
+
+_lib = None
+
+
+class EmtdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"emtd error 0x{code:04x}: {msg}")
+        self.code = code
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("kuzlov", C.c_int), ("kpar", C.c_int), ("n_edges", C.c_int), ("n_materials", C.c_int),
+                ("xyz", C.c_void_p), ("nver", C.c_void_p), ("ed", C.c_void_p), ("edges", C.c_void_p),
+                ("nvkat", C.c_void_p), ("sigma", C.c_void_p), ("mu", C.c_void_p)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("n_factored", C.c_int64), ("n_fronts", C.c_int64), ("n_levels", C.c_int64),
+                ("nnz_l", C.c_double), ("factor_flops", C.c_double), ("factor_seconds", C.c_double),
+                ("solve_seconds", C.c_double), ("refine_iterations", C.c_int64), ("fallbacks", C.c_int64),
+                ("last_residual", C.c_double)]
+
+
+class CalcInfo(C.Structure):
+    _fields_ = [("n_edges", C.c_int64), ("n_elements", C.c_int64), ("n_nodes", C.c_int64), ("npls", C.c_int64),
+                ("ntime", C.c_int64), ("n_decades", C.c_int64), ("t_first", C.c_int64), ("t_last", C.c_int64),
+                ("n_rec_b", C.c_int64), ("n_rec_e", C.c_int64), ("setup_seconds", C.c_double),
+                ("factor_seconds", C.c_double), ("step_seconds", C.c_double), ("steps", C.c_int64),
+                ("nnz_l", C.c_double), ("factor_flops", C.c_double), ("refine_iterations", C.c_int64),
+                ("fallbacks", C.c_int64)]
+
+
+EXPORTS = [
+    "emtd_last_error", "emtd_portrait", "emtd_assemble",
+    "emtd_solver_create", "emtd_solver_factorize", "emtd_solver_solve_nrhs", "emtd_solver_stop",
+    "emtd_solver_destroy", "emtd_solver_get_stats",
+    "emtd_calc_create", "emtd_calc_run", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
+    "emtd_calc_last_field", "emtd_calc_get_info", "emtd_calc_destroy", "emtd_calc_nonstat_3d_line",
+]
+
+
+def lib():
+    """The loaded native library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"native library missing: {LIB_PATH} (run the package's build.py)")
+        L = C.CDLL(LIB_PATH)
+        L.emtd_last_error.restype = C.c_char_p
+        L.emtd_solver_destroy.restype = None
+        L.emtd_calc_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EmtdError(rc, lib().emtd_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Mesh:
+    """Arrays of a brick mesh (0-based), kept alive for the C struct."""
+
+    def __init__(self, xyz, nver, ed, edges, nvkat, sigma, mu):
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.nver = np.ascontiguousarray(nver[:, :8], dtype=np.int32)
+        self.ed = np.ascontiguousarray(ed[:, :12], dtype=np.int32)
+        self.edges = np.ascontiguousarray(edges, dtype=np.int32)
+        self.nvkat = np.ascontiguousarray(nvkat, dtype=np.int32)
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.mu = np.ascontiguousarray(mu, dtype=np.float64)
+        self.c = _Mesh(len(self.xyz), len(self.nver), len(self.edges), len(self.sigma), _p(self.xyz),
+                       _p(self.nver), _p(self.ed), _p(self.edges), _p(self.nvkat), _p(self.sigma), _p(self.mu))
+
+
+def portrait(mesh: Mesh):
+    """Lower RSF pattern (ig, jg) of the edge-element matrix."""
+    n = len(mesh.edges)
+    ig = np.zeros(n + 1, dtype=np.int32)
+    nnz = C.c_int64(0)
+    _check(lib().emtd_portrait(C.byref(mesh.c), _p(ig), C.byref(nnz), None))
+    jg = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    _check(lib().emtd_portrait(C.byref(mesh.c), _p(ig), C.byref(nnz), _p(jg)))
+    return ig, jg[:nnz.value]
+
+
+def assemble(mesh: Mesh, ig, jg, device: int = -1):
+    """(b_ggl, b_di, c_ggl, c_di) assembled on the GPU."""
+    ig = np.ascontiguousarray(ig, dtype=np.int32)
+    jg = np.ascontiguousarray(jg, dtype=np.int32)
+    n = len(ig) - 1
+    nnz = int(ig[-1])
+    out = [np.zeros(max(nnz, 1)), np.zeros(n), np.zeros(max(nnz, 1)), np.zeros(n)]
+    _check(lib().emtd_assemble(C.byref(mesh.c), _p(ig), _p(jg), C.c_int(device), *[_p(a) for a in out]))
+    return out[0][:nnz], out[1], out[2][:nnz], out[3]
+
+
+class Solver:
+    """GPU replacement of ``pardiso_solver``: factorize (phase 12), solve_nrhs (phase 33)."""
+
+    def __init__(self, device: int = -1, precision: int = FP64):
+        self.h = C.c_void_p()
+        _check(lib().emtd_solver_create(C.c_int(device), C.c_int(precision), C.byref(self.h)))
+
+    def factorize(self, ig, jg, ggl, di):
+        self.ig = np.ascontiguousarray(ig, dtype=np.int32)
+        self.jg = np.ascontiguousarray(jg, dtype=np.int32)
+        ggl = np.ascontiguousarray(ggl, dtype=np.float64)
+        di = np.ascontiguousarray(di, dtype=np.float64)
+        self.n = len(di)
+        _check(lib().emtd_solver_factorize(self.h, C.c_int(self.n), _p(self.ig), _p(self.jg), _p(ggl), _p(di)))
+
+    def solve(self, b):
+        b = np.asarray(b, dtype=np.float64)
+        cols = b.reshape(-1, self.n) if b.ndim == 2 else b.reshape(1, -1)
+        cols = np.ascontiguousarray(cols)
+        x = np.zeros_like(cols)
+        _check(lib().emtd_solver_solve_nrhs(self.h, C.c_int(cols.shape[0]), _p(cols), _p(x)))
+        return x.reshape(b.shape)
+
+    def stats(self) -> SolverStats:
+        st = SolverStats()
+        _check(lib().emtd_solver_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().emtd_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Calc:
+    """The calculation module on one working directory (reference files in, tables out)."""
+
+    def __init__(self, workdir: str, form: int = -1, scheme: int = -1, precision: int = FP64, device: int = -1):
+        self.h = C.c_void_p()
+        _check(lib().emtd_calc_create(workdir.encode(), C.c_int(form), C.c_int(scheme), C.c_int(precision),
+                                      C.c_int(device), C.byref(self.h)))
+
+    def run(self):
+        _check(lib().emtd_calc_run(self.h))
+
+    def write_outputs(self):
+        _check(lib().emtd_calc_write_outputs(self.h))
+
+    def info(self) -> CalcInfo:
+        inf = CalcInfo()
+        _check(lib().emtd_calc_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def receiver_values(self, kind: int):
+        inf = self.info()
+        nrec = inf.n_rec_b if kind == 0 else inf.n_rec_e
+        nt = inf.t_last - inf.t_first + 1
+        out = np.zeros((nt, inf.npls, nrec, 3))
+        _check(lib().emtd_calc_receiver_values(self.h, C.c_int(kind), _p(out)))
+        return out
+
+    def last_field(self):
+        inf = self.info()
+        out = np.zeros((inf.npls, inf.n_edges))
+        _check(lib().emtd_calc_last_field(self.h, _p(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().emtd_calc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calc_nonstat_3d_line(workdir: str, form: int = -1, scheme: int = -1, precision: int = FP64,
+                         device: int = -1):
+    """Run the module on ``workdir`` exactly as ``CalcNonstat3DLine.exe`` would."""
+    _check(lib().emtd_calc_nonstat_3d_line(workdir.encode(), C.c_int(form), C.c_int(scheme),
+                                           C.c_int(precision), C.c_int(device)))

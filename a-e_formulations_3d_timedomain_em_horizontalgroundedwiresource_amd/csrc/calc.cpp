@@ -1,0 +1,559 @@
+// The calculation module on the GPU: CalcSP + Time_approx_for_vfem of the reference
+// (A2/TaskSP.cpp:445-711, A2/time_approx.cpp:289-1152 and the A3/E2/E3 deltas).
+//
+// Everything that runs per step stays on the device: the edge fields of the last layers
+// (u_j, u_j1, u_j2 and the layers kept for the next decade's restart), the factor, the RHS
+// and the receiver values. Vectors are held in the solver's permuted order; the Dirichlet
+// edges are the trailing permuted indices and stay zero after the first step.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/emtd.h"
+#include "assemble.hpp"
+#include "calc_core.hpp"
+#include "chol.hpp"
+#include "common.hpp"
+#include "fem_host.hpp"
+#include "model_io.hpp"
+#include "step.hpp"
+#include "symbolic.hpp"
+#include "vec.hpp"
+
+using namespace emtd;
+
+namespace emtd {
+
+CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, int device)
+    : dir_(dir), device_(device), precision_(precision) {
+    const double t0 = wall_seconds();
+    select_device(device);
+    EMTD_HIP(hipStreamCreate(&stream_));
+    M_ = read_model_files(dir, form);
+    form_ = form >= 0 ? form : (M_.variant_form >= 0 ? M_.variant_form : 0);
+    scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
+    if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
+    n_ = M_.n;
+    npls_ = M_.npls;
+    // pattern and GPU assembly of B and C (original numbering)
+    build_portrait(M_.kpar, M_.ed.data(), n_, ig_, jg_);
+    nnz_ = ig_[n_];
+    emtd_mesh mesh{M_.kuzlov, M_.kpar, n_, M_.nmat, M_.xyz.data(), M_.nver.data(), M_.ed.data(),
+                   M_.edges.data(), M_.nvkat.data(), M_.sigma.data(), M_.mu.data()};
+    b_ggl_.alloc(std::max<int64_t>(nnz_, 1));
+    c_ggl_.alloc(std::max<int64_t>(nnz_, 1));
+    b_di_.alloc(n_);
+    c_di_.alloc(n_);
+    {
+        Assembler A(mesh, n_, ig_.data(), jg_.data(), stream_);
+        A.run(b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p);
+        EMTD_HIP(hipStreamSynchronize(stream_));
+    }
+    // ordering and symbolic analysis; Dirichlet edges are singletons
+    bnd_ = boundary_edge_mask(M_);
+    std::vector<char> active(n_);
+    for (int i = 0; i < n_; ++i) active[i] = !bnd_[i];
+    std::vector<GridCoord> coords = edge_grid_coords(M_.kuzlov, M_.xyz.data(), n_, M_.edges.data());
+    OrderingOptions opt;
+    S_ = std::make_unique<Symbolic>(analyze(n_, ig_.data(), jg_.data(), active, coords, opt));
+    nn_ = S_->nn;
+    {
+        std::vector<int64_t> doff, ddiag;
+        assembly_map(*S_, n_, ig_.data(), jg_.data(), doff, ddiag);
+        dest_off_ = to_device(doff, stream_);
+        dest_diag_ = to_device(ddiag, stream_);
+    }
+    if (precision_ == EMTD_FP64)
+        c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
+    else
+        c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
+    perm_ = to_device(S_->perm, stream_);
+    // C in permuted full CSR (exact zeros dropped) for the RHS
+    {
+        std::vector<double> cg(nnz_), cd(n_);
+        c_ggl_.download(cg.data(), nnz_, stream_);
+        c_di_.download(cd.data(), n_, stream_);
+        EMTD_HIP(hipStreamSynchronize(stream_));
+        c_host_ggl_ = cg;
+        c_host_di_ = cd;
+        std::vector<std::vector<std::pair<int, double>>> rows(n_);
+        for (int i = 0; i < n_; ++i) {
+            const int a = S_->iperm[i];
+            if (cd[i] != 0.0) rows[a].push_back({a, cd[i]});
+            for (int k = ig_[i]; k < ig_[i + 1]; ++k) {
+                if (cg[k] == 0.0) continue;
+                const int b = S_->iperm[jg_[k]];
+                rows[a].push_back({b, cg[k]});
+                rows[b].push_back({a, cg[k]});
+            }
+        }
+        std::vector<int64_t> rp(n_ + 1, 0);
+        std::vector<int> ci;
+        std::vector<double> cv;
+        for (int a = 0; a < n_; ++a) {
+            std::sort(rows[a].begin(), rows[a].end());
+            for (auto& e : rows[a]) { ci.push_back(e.first); cv.push_back(e.second); }
+            rp[a + 1] = int64_t(ci.size());
+        }
+        crp_ = to_device(rp, stream_);
+        cci_ = to_device(ci, stream_);
+        ccv_ = to_device(cv, stream_);
+        c_nnz_ = int64_t(ci.size());
+    }
+    // initial field (A: nodal a0 projected on edges; E: -grad V), permuted
+    {
+        std::vector<double> u0(size_t(n_) * npls_, 0.0);
+        for (int ip = 0; ip < npls_; ++ip)
+            for (int i = 0; i < n_; ++i) {
+                const int v1 = M_.edges[2 * i], v2 = M_.edges[2 * i + 1];
+                double axis[3], len = 0;
+                for (int a = 0; a < 3; ++a) {
+                    axis[a] = M_.xyz[3 * v2 + a] - M_.xyz[3 * v1 + a];
+                    len += axis[a] * axis[a];
+                }
+                len = std::sqrt(len);
+                double u;
+                if (form_ == EMTD_FORM_A) {
+                    const double* a0 = M_.a0.data() + size_t(ip) * M_.kuzlov * 3;
+                    double s = 0;
+                    for (int a = 0; a < 3; ++a) s += 0.5 * (a0[3 * v1 + a] + a0[3 * v2 + a]) * axis[a];
+                    u = s / len * len * 0.5;
+                } else {
+                    const double* V = M_.v3a.data() + size_t(ip) * M_.kuzlov;
+                    u = -(V[v2] - V[v1]) / len * len * 0.5;
+                }
+                u0[size_t(ip) * n_ + S_->iperm[i]] = u;
+            }
+        u0_ = to_device(u0, stream_);
+    }
+    // line sources
+    {
+        SourceInfo si = place_line_sources(M_);
+        if (!si.shape.empty()) {
+            std::vector<double> sp(si.shape.size());
+            for (int ip = 0; ip < npls_; ++ip)
+                for (int i = 0; i < n_; ++i) sp[size_t(ip) * n_ + S_->iperm[i]] = si.shape[size_t(ip) * n_ + i];
+            src_ = to_device(sp, stream_);
+            src_info_ = si;
+        }
+    }
+    // receivers: one functional table for both kinds, rows (kind, receiver, component)
+    {
+        ReceiverOps rb = build_receiver_ops(M_, M_.recB, 0);
+        ReceiverOps re = build_receiver_ops(M_, M_.recE, 1);
+        nrecB_ = rb.nrec;
+        nrecE_ = re.nrec;
+        std::vector<int64_t> rp{0};
+        std::vector<int> idx;
+        std::vector<double> w;
+        for (const ReceiverOps* R : {&rb, &re}) {
+            for (size_t k = 0; k + 1 < R->rp.size(); ++k) {
+                for (int64_t j = R->rp[k]; j < R->rp[k + 1]; ++j) {
+                    idx.push_back(S_->iperm[R->idx[j]]);
+                    w.push_back(R->w[j]);
+                }
+                rp.push_back(int64_t(idx.size()));
+            }
+        }
+        rec_rows_ = int(rp.size()) - 1;
+        rec_rp_ = to_device(rp, stream_);
+        rec_idx_ = to_device(idx.empty() ? std::vector<int>{0} : idx, stream_);
+        rec_w_ = to_device(w.empty() ? std::vector<double>{0.0} : w, stream_);
+    }
+    print_window(M_, first_, last_);
+    plan_ = decade_plan(M_.times);
+    const int nt_out = last_ - first_ + 1;
+    rec_vals_.alloc(size_t(std::max(nt_out, 1)) * npls_ * std::max(rec_rows_, 1));
+    EMTD_HIP(hipMemsetAsync(rec_vals_.p, 0, rec_vals_.n * sizeof(double), stream_));
+    rec_done_.assign(M_.times.size(), 0);
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    setup_seconds_ = wall_seconds() - t0;
+}
+
+CalcCore::~CalcCore() {
+    try {
+        select_device(device_);
+        c64_.reset();
+        c32_.reset();
+        stored_.clear();
+        if (stream_) (void)hipStreamDestroy(stream_);
+    } catch (...) {
+    }
+}
+
+bool CalcCore::stop_requested() const { return file_exists(join_path(dir_, "stop")); }
+
+void CalcCore::factorize(double mult) {
+    const double t0 = wall_seconds();
+    int bad;
+    if (c64_) {
+        c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
+        bad = c64_->check_info();
+    } else {
+        c32_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
+        bad = c32_->check_info();
+    }
+    if (bad >= 0) throw Error(EMTD_ERR_NOTSPD, "non-positive pivot at permuted index " + std::to_string(bad));
+    factor_seconds_ += wall_seconds() - t0;
+    ++n_factorizations_;
+}
+
+void CalcCore::solve(const double* b, double* x) {
+    if (c64_)
+        c64_->solve(b, x, npls_, n_);
+    else
+        c32_->solve(b, x, npls_, n_);
+    launch_zero_tail(stream_, x, nn_, n_, npls_);
+}
+
+double CalcCore::source_scale(int itime, double dt_two_layer) const {
+    if (src_.n == 0) return 0.0;
+    const std::vector<double>& cur = M_.current;
+    const std::vector<double>& t = M_.times;
+    if (form_ == EMTD_FORM_A) return cur[itime];  // injected only when nonzero (A2/time_approx.cpp:1041)
+    // E: -dI/dt (E2/time_approx.cpp:755-775)
+    const int nt = int(t.size());
+    const double dtb = dt_two_layer > 0 ? dt_two_layer : (t[itime] - t[itime - 1]);
+    double djdt;
+    if (itime + 1 < nt) {
+        if (cur[itime] == 0.0) {
+            djdt = (cur[itime] - cur[itime - 1]) / dtb;
+        } else {
+            // Time_approx_for_vfem::dA_dt called with (t, I[i-1], I[i], I[i+1], ...), literally
+            double tt = t[itime];
+            const double uj = cur[itime - 1], uj1 = cur[itime], uj2 = cur[itime + 1];
+            const double dt = t[itime + 1] - t[itime - 1], dt0 = t[itime + 1] - t[itime], dt1 = t[itime] - t[itime - 1];
+            const double tj = t[itime], tj1 = t[itime - 1], tj2 = itime >= 2 ? t[itime - 2] : t[0];
+            if (tt < tj2) tt = tj2;
+            if (tt > tj) tt = tj;
+            djdt = uj2 * (2.0 * tt - tj - tj1) / (dt1 * dt) - uj1 * (2.0 * tt - tj - tj2) / (dt1 * dt0) +
+                   uj * (2.0 * tt - tj1 - tj2) / (dt * dt0);
+        }
+    } else {
+        djdt = (cur[itime] - cur[itime - 1]) / dtb;
+    }
+    return -djdt;
+}
+
+void CalcCore::output_layer(int layer, const double* u) {
+    if (layer < first_ || layer > last_ || rec_rows_ == 0) return;
+    double* out = rec_vals_.p + size_t(layer - first_) * npls_ * rec_rows_;
+    launch_receivers(stream_, rec_rows_, npls_, rec_rp_.p, rec_idx_.p, rec_w_.p, u, n_, out);
+    rec_done_[layer] = 1;
+}
+
+double* CalcCore::keep_layer(int layer, const double* u) {
+    auto it = stored_.find(layer);
+    if (it == stored_.end()) it = stored_.emplace(layer, DevBuf<double>(size_t(n_) * npls_)).first;
+    EMTD_HIP(hipMemcpyAsync(it->second.p, u, size_t(n_) * npls_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    return it->second.p;
+}
+
+void CalcCore::run() {
+    const std::vector<double>& t = M_.times;
+    const int nt = int(t.size());
+    const size_t vn = size_t(n_) * npls_;
+    DevBuf<double> bufA(vn), bufB(vn), bufC(vn), rhs(vn);
+    double* uj = bufA.p;
+    double* uj1 = bufB.p;
+    double* uj2 = bufC.p;
+    vpre1_.assign(nt, -1);
+    vpre2_.assign(nt, -2);
+    const double ts = wall_seconds();
+    double t_factor0 = factor_seconds_;
+    for (size_t k = 0; k < plan_.size(); ++k) {
+        const Decade& D = plan_[k];
+        double t_j = 0, t_j1 = 0, t_j2 = 0;
+        int tn = 0, tn1 = 0, tn2 = 0;
+        int start;
+        if (1 >= D.tbeg && 1 < D.tend) {
+            for (double* p : {uj, uj1, uj2})
+                EMTD_HIP(hipMemcpyAsync(p, u0_.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            t_j = t_j1 = t[0];
+            tn = tn1 = 0;
+            if (scheme_ == 3) {
+                // Two_Layers_Schema: steps 1 and 2 with dt = t1 - t0, no receiver output
+                const double dtl = t[1] - t[0];
+                factorize(1.0 / dtl);
+                for (int it = 1; it <= 2 && it < nt; ++it) {
+                    launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, uj, uj, 1.0 / dtl, 0.0,
+                               src_.n ? src_.p : nullptr, source_scale(it, dtl), rhs.p);
+                    solve(rhs.p, uj2);  // solution into the free buffer
+                    std::swap(uj, uj2);  // uj = new solution, uj2 = previous u_j
+                    if (it >= D.nlst) keep_layer(it, uj);
+                    vpre1_[it] = it - 1;
+                    vpre2_[it] = it - 2;
+                    // shift: u_j2 <- u_j1, u_j1 <- u_j
+                    EMTD_HIP(hipMemcpyAsync(uj2, uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                    EMTD_HIP(hipMemcpyAsync(uj1, uj, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                    ++steps_;
+                }
+                t_j2 = t[0];
+                t_j1 = t[1];
+                t_j = t[2];
+                tn2 = 0;
+                tn1 = 1;
+                tn = 2;
+                start = 3;
+            } else {
+                start = 1;
+            }
+        } else {
+            start = D.tbeg;
+        }
+        bool fcalc = false, first_step = true;
+        double mult = 0, mult1 = 0, mult2 = 0, dt = 0, dt0 = 0, dt1 = 0;
+        for (int it = std::max(start, D.tbeg); it < D.tend; ++it) {
+            if (stop_requested()) throw Error(EMTD_ERR_STOPPED, "stop file found");
+            bool fsdiff = false;
+            if (!fcalc && it > (scheme_ == 2 ? 1 : 3)) {
+                auto a = stored_.find(D.npre1), b = stored_.find(D.npre2);
+                if (a == stored_.end() || b == stored_.end()) throw Error(EMTD_ERR_NOFILE, "restart layer missing");
+                EMTD_HIP(hipMemcpyAsync(uj1, a->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                EMTD_HIP(hipMemcpyAsync(uj2, b->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                fsdiff = true;
+            }
+            if (!fsdiff) {
+                t_j2 = t_j1; t_j1 = t_j; t_j = t[it];
+                tn2 = tn1; tn1 = tn; tn = it;
+            } else {
+                t_j2 = t[D.npre2]; t_j1 = t[D.npre1]; t_j = t[it];
+                tn2 = D.npre2; tn1 = D.npre1; tn = it;
+            }
+            vpre1_[it] = tn1;
+            vpre2_[it] = tn2;
+            if (first_step) {
+                dt = t_j - t_j2;
+                dt0 = t_j - t_j1;
+                dt1 = t_j1 - t_j2;
+                if (scheme_ == 2) {
+                    mult = 1.0 / (t_j - t_j1);
+                    mult1 = mult;
+                    mult2 = 0.0;
+                }
+                first_step = false;
+            }
+            if (scheme_ == 3) {
+                mult = (dt + dt0) / (dt * dt0);
+                mult1 = dt / (dt1 * dt0);
+                mult2 = -dt0 / (dt * dt1);
+            }
+            if (!fcalc) factorize(mult);
+            launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, uj1, uj2, mult1, mult2,
+                       src_.n ? src_.p : nullptr, source_scale(it, -1.0), rhs.p);
+            solve(rhs.p, uj);
+            if (it >= D.nlst) keep_layer(it, uj);
+            if (tn1 < last_) output_layer(tn1, uj1);
+            if (tn == last_) output_layer(tn, uj);
+            // u_j2 <- u_j1, u_j1 <- u_j (pointer rotation; the old u_j2 buffer is reused)
+            double* tmp = uj2;
+            uj2 = uj1;
+            uj1 = uj;
+            uj = tmp;
+            fcalc = true;
+            ++steps_;
+            last_layer_ = it;
+        }
+        // layers kept for restarts that no later decade can need
+        for (auto itS = stored_.begin(); itS != stored_.end();) {
+            bool needed = false;
+            for (size_t k2 = k + 1; k2 < plan_.size(); ++k2)
+                if (itS->first == plan_[k2].npre1 || itS->first == plan_[k2].npre2) needed = true;
+            if (!needed && itS->first != last_layer_) itS = stored_.erase(itS);
+            else ++itS;
+        }
+    }
+    // the last field (a.last, A2/time_approx.cpp:1105-1111) is the newest u_j1
+    last_field_.alloc(vn);
+    EMTD_HIP(hipMemcpyAsync(last_field_.p, uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    const double total = wall_seconds() - ts;
+    step_seconds_ = total - (factor_seconds_ - t_factor0);
+    ran_ = true;
+}
+
+void CalcCore::receiver_values(int kind, double* out) const {
+    if (!ran_) throw Error(EMTD_ERR_NOTINIT, "run before reading receivers");
+    const int nt_out = last_ - first_ + 1;
+    std::vector<double> all(size_t(nt_out) * npls_ * rec_rows_);
+    if (!all.empty()) {
+        EMTD_HIP(hipMemcpy(all.data(), rec_vals_.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    const int nrec = kind == 0 ? nrecB_ : nrecE_;
+    const int row0 = kind == 0 ? 0 : 3 * nrecB_;
+    for (int l = 0; l < nt_out; ++l)
+        for (int ip = 0; ip < npls_; ++ip)
+            for (int r = 0; r < nrec; ++r)
+                for (int c = 0; c < 3; ++c)
+                    out[((int64_t(l) * npls_ + ip) * nrec + r) * 3 + c] =
+                        rec_done_[first_ + l] ? all[(size_t(l) * npls_ + ip) * rec_rows_ + row0 + 3 * r + c] : 0.0;
+}
+
+void CalcCore::last_field(double* out) const {
+    if (!ran_) throw Error(EMTD_ERR_NOTINIT, "run before reading the field");
+    std::vector<double> v(size_t(n_) * npls_);
+    EMTD_HIP(hipMemcpy(v.data(), last_field_.p, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int ip = 0; ip < npls_; ++ip)
+        for (int i = 0; i < n_; ++i) out[size_t(ip) * n_ + i] = v[size_t(ip) * n_ + S_->iperm[i]];
+}
+
+void CalcCore::write_outputs() const {
+    const int nt_out = last_ - first_ + 1;
+    std::vector<double> B(size_t(nt_out) * npls_ * nrecB_ * 3), E(size_t(nt_out) * npls_ * nrecE_ * 3);
+    receiver_values(0, B.data());
+    receiver_values(1, E.data());
+    std::vector<double> eds(B.size()), efield(E.size());
+    if (form_ == EMTD_FORM_A) {
+        if (nrecB_) diff_series(M_.times, B.data(), eds.data(), first_, last_, npls_ * nrecB_ * 3, vpre1_, vpre2_, scheme_);
+        if (nrecE_) {
+            diff_series(M_.times, E.data(), efield.data(), first_, last_, npls_ * nrecE_ * 3, vpre1_, vpre2_, scheme_);
+            for (double& v : efield) v = -v;
+        }
+        write_tables(dir_, M_, first_, last_, eds.data(), B.data(), efield.data());
+    } else {
+        for (size_t i = 0; i < B.size(); ++i) eds[i] = -B[i];
+        write_tables(dir_, M_, first_, last_, eds.data(), nullptr, E.data());
+    }
+    // decade files of GenerateDecInfo (A2/TaskSP.cpp:351-398)
+    FILE* f = std::fopen(join_path(dir_, "nDec").c_str(), "w");
+    if (f) { std::fprintf(f, "%d\n", int(plan_.size())); std::fclose(f); }
+    f = std::fopen(join_path(dir_, "DecIg").c_str(), "w");
+    if (f) {
+        std::fprintf(f, "0\n");
+        for (const Decade& D : plan_) std::fprintf(f, "%d\n", D.tend);
+        std::fclose(f);
+    }
+    int mx = 0;
+    for (const Decade& D : plan_) mx = std::max(mx, D.tend - D.tbeg);
+    f = std::fopen(join_path(dir_, "max_dec_size").c_str(), "w");
+    if (f) { std::fprintf(f, "%d\n", mx); std::fclose(f); }
+    // a.last: the last field in original numbering
+    std::vector<double> lf(size_t(n_) * npls_);
+    last_field(lf.data());
+    f = std::fopen(join_path(dir_, "a.last").c_str(), "wb");
+    if (f) { std::fwrite(lf.data(), sizeof(double), lf.size(), f); std::fclose(f); }
+    if (!src_info_.elem.empty()) {
+        f = std::fopen(join_path(dir_, "sours_dip_calc").c_str(), "w");
+        if (f) {
+            for (size_t j = 0; j < src_info_.elem.size(); ++j)
+                std::fprintf(f, "%g %g %g %g %g %g %d %d \n", src_info_.pos[3 * j], src_info_.pos[3 * j + 1],
+                             src_info_.pos[3 * j + 2], src_info_.loc[3 * j], src_info_.loc[3 * j + 1],
+                             src_info_.loc[3 * j + 2], src_info_.elem[j] + 1, M_.nvkat[src_info_.elem[j]] + 1);
+            std::fclose(f);
+        }
+    }
+}
+
+void CalcCore::info(emtd_calc_info* inf) const {
+    inf->n_edges = n_;
+    inf->n_elements = M_.kpar;
+    inf->n_nodes = M_.kuzlov;
+    inf->npls = npls_;
+    inf->ntime = int64_t(M_.times.size());
+    inf->n_decades = int64_t(plan_.size());
+    inf->t_first = first_;
+    inf->t_last = last_;
+    inf->n_rec_b = nrecB_;
+    inf->n_rec_e = nrecE_;
+    inf->setup_seconds = setup_seconds_;
+    inf->factor_seconds = factor_seconds_;
+    inf->step_seconds = step_seconds_;
+    inf->steps = steps_;
+    inf->nnz_l = S_->nnz_l;
+    inf->factor_flops = S_->flops;
+    inf->refine_iterations = refine_iterations_;
+    inf->fallbacks = fallbacks_;
+}
+
+}  // namespace emtd
+
+// ------------------------------------------------------------------ C ABI
+struct emtd_calc {
+    std::unique_ptr<CalcCore> core;
+};
+
+#define CALC_GUARD_BEGIN try {
+#define CALC_GUARD_END                           \
+    }                                            \
+    catch (const Error& e) {                     \
+        set_last_error(e.what());                \
+        return e.code;                           \
+    }                                            \
+    catch (const std::bad_alloc&) {              \
+        set_last_error("host allocation failed"); \
+        return EMTD_ERR_NOMEM;                   \
+    }                                            \
+    catch (const std::exception& e) {            \
+        set_last_error(e.what());                \
+        return EMTD_ERR_ERROR;                   \
+    }
+
+extern "C" int emtd_calc_create(const char* workdir, int formulation, int scheme, int precision, int device,
+                                emtd_calc** out) {
+    CALC_GUARD_BEGIN
+    if (!workdir || !out) throw Error(EMTD_ERR_RANGE, "null argument");
+    if (precision != EMTD_FP64 && precision != EMTD_FP32_IR) throw Error(EMTD_ERR_RANGE, "unknown precision");
+    auto c = std::make_unique<emtd_calc>();
+    c->core = std::make_unique<CalcCore>(workdir, formulation, scheme, precision, device);
+    *out = c.release();
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_run(emtd_calc* c) {
+    CALC_GUARD_BEGIN
+    if (!c) throw Error(EMTD_ERR_RANGE, "null argument");
+    c->core->run();
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_write_outputs(emtd_calc* c) {
+    CALC_GUARD_BEGIN
+    if (!c) throw Error(EMTD_ERR_RANGE, "null argument");
+    c->core->write_outputs();
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_receiver_values(const emtd_calc* c, int kind, double* out) {
+    CALC_GUARD_BEGIN
+    if (!c || !out || (kind != 0 && kind != 1)) throw Error(EMTD_ERR_RANGE, "bad argument");
+    c->core->receiver_values(kind, out);
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_last_field(const emtd_calc* c, double* out) {
+    CALC_GUARD_BEGIN
+    if (!c || !out) throw Error(EMTD_ERR_RANGE, "bad argument");
+    c->core->last_field(out);
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info) {
+    CALC_GUARD_BEGIN
+    if (!c || !info) throw Error(EMTD_ERR_RANGE, "bad argument");
+    c->core->info(info);
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" void emtd_calc_destroy(emtd_calc* c) { delete c; }
+
+extern "C" int emtd_calc_nonstat_3d_line(const char* workdir, int formulation, int scheme, int precision,
+                                         int device) {
+    emtd_calc* c = nullptr;
+    int rc = emtd_calc_create(workdir, formulation, scheme, precision, device, &c);
+    if (rc == EMTD_OK) rc = emtd_calc_run(c);
+    if (rc == EMTD_OK) rc = emtd_calc_write_outputs(c);
+    emtd_calc_destroy(c);
+    return rc;
+}

@@ -1,0 +1,73 @@
+// The calculation module's state and time loop (implementation in calc.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/emtd.h"
+#include "chol.hpp"
+#include "common.hpp"
+#include "fem_host.hpp"
+#include "model_io.hpp"
+#include "symbolic.hpp"
+
+namespace emtd {
+
+class CalcCore {
+  public:
+    CalcCore(const std::string& dir, int form, int scheme, int precision, int device);
+    ~CalcCore();
+    void run();
+    void receiver_values(int kind, double* out) const;
+    void last_field(double* out) const;
+    void write_outputs() const;
+    void info(emtd_calc_info* inf) const;
+
+  private:
+    bool stop_requested() const;
+    void factorize(double mult);
+    void solve(const double* b, double* x);
+    double source_scale(int itime, double dt_two_layer) const;
+    void output_layer(int layer, const double* u);
+    double* keep_layer(int layer, const double* u);
+
+    std::string dir_;
+    int device_ = -1, precision_ = 0, form_ = 0, scheme_ = 2;
+    hipStream_t stream_ = nullptr;
+    ModelFiles M_;
+    int n_ = 0, nn_ = 0, npls_ = 1;
+    int64_t nnz_ = 0, c_nnz_ = 0;
+    std::vector<int> ig_, jg_;
+    std::vector<char> bnd_;
+    std::vector<double> c_host_ggl_, c_host_di_;
+    DevBuf<double> b_ggl_, b_di_, c_ggl_, c_di_;
+    std::unique_ptr<Symbolic> S_;
+    DevBuf<int64_t> dest_off_, dest_diag_;
+    std::unique_ptr<GpuCholesky<double>> c64_;
+    std::unique_ptr<GpuCholesky<float>> c32_;
+    DevBuf<int> perm_;
+    DevBuf<int64_t> crp_;
+    DevBuf<int> cci_;
+    DevBuf<double> ccv_;
+    DevBuf<double> u0_, src_;
+    SourceInfo src_info_;
+    int nrecB_ = 0, nrecE_ = 0, rec_rows_ = 0;
+    DevBuf<int64_t> rec_rp_;
+    DevBuf<int> rec_idx_;
+    DevBuf<double> rec_w_, rec_vals_;
+    std::vector<char> rec_done_;
+    int first_ = 0, last_ = 0, last_layer_ = 0;
+    std::vector<Decade> plan_;
+    std::vector<int> vpre1_, vpre2_;
+    std::map<int, DevBuf<double>> stored_;
+    DevBuf<double> last_field_;
+    bool ran_ = false;
+    double setup_seconds_ = 0, factor_seconds_ = 0, step_seconds_ = 0;
+    int64_t steps_ = 0, n_factorizations_ = 0, refine_iterations_ = 0, fallbacks_ = 0;
+};
+
+}  // namespace emtd

@@ -1,0 +1,67 @@
+// GPU supernodal multifrontal Cholesky: numeric factorization and triangular solves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "symbolic.hpp"
+
+namespace emtd {
+
+// Per-front descriptor as the kernels read it.
+struct DevFront {
+    int64_t loff;       // L_f: m x p column-major (ld = m)
+    int64_t uoff;       // U_f: u x u column-major (ld = u)
+    int64_t woff;       // solve workspace: m entries
+    int64_t rows_off;   // rows of the front (m entries, permuted indices)
+    int64_t relmap_off; // this front's update rows inside its parent (u entries)
+    int fbeg, p, m, parent;
+    int child_beg, child_end;
+};
+
+// Device copy of the symbolic structure plus factor storage of type T (double or float).
+template <typename T>
+class GpuCholesky {
+  public:
+    GpuCholesky(const Symbolic& S, hipStream_t stream);
+
+    // Numeric factorization of  mult*C + B  (+1e-12 diagonal bump, Dirichlet rows dropped) from
+    // the RSF value arrays of B and C on the device (original numbering, lower pattern).
+    void factorize_system(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
+                          const double* d_b_ggl, const double* d_b_di, const double* d_c_ggl,
+                          const double* d_c_di, double mult);
+    // Numeric factorization of a given lower RSF matrix (ggl, di) on the device.
+    void factorize_values(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
+                          const double* d_ggl, const double* d_di);
+    // Solve in the permuted space: b and x hold nn*nrhs entries (column stride ld).
+    void solve(const double* d_b, double* d_x, int nrhs, int64_t ld);
+
+    // Returns -1 or the first (permuted) pivot that was not positive; synchronizes.
+    int check_info();
+    double last_factor_seconds() const { return factor_seconds_; }
+
+    const Symbolic& sym() const { return S_; }
+    hipStream_t stream() const { return stream_; }
+
+  private:
+    void numeric();
+    const Symbolic& S_;
+    hipStream_t stream_;
+    DevBuf<T> L_, U_;
+    DevBuf<double> W_;
+    DevBuf<DevFront> fronts_;
+    DevBuf<int> rows_, children_, relmap_, level_fronts_;
+    DevBuf<int2> ea_items_;             // extend-add work items (child, column)
+    std::vector<int> ea_ptr_;           // per (level, pass) range into ea_items_
+    std::vector<std::vector<int>> level_sorted_;  // fronts of a level by decreasing p
+    DevBuf<int> level_sorted_dev_;
+    std::vector<int> level_sorted_off_;
+    DevBuf<int> info_;
+    int wrhs_ = 0;
+    double factor_seconds_ = 0;
+};
+
+}  // namespace emtd

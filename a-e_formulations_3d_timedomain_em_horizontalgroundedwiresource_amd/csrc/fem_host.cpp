@@ -1,0 +1,491 @@
+// Host-side setup of the calculation module (runs once per model) and its output tables.
+#include "fem_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+
+#include "common.hpp"
+
+namespace emtd {
+
+void build_portrait(int kpar, const int* ed, int n, std::vector<int>& ig, std::vector<int>& jg) {
+    std::vector<int> cnt(n + 1, 0);
+    // rows gather candidates; duplicates removed per row
+    std::vector<std::vector<int>> rows(n);
+    for (int e = 0; e < kpar; ++e) {
+        const int* E = ed + 12 * e;
+        for (int a = 0; a < 12; ++a)
+            for (int b = 0; b < 12; ++b)
+                if (E[b] < E[a]) rows[E[a]].push_back(E[b]);
+    }
+    ig.assign(n + 1, 0);
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        std::vector<int>& r = rows[i];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        total += int64_t(r.size());
+        if (total > INT32_MAX) throw Error(0x0004, "pattern exceeds 32-bit RSF indices");
+        ig[i + 1] = int(total);
+    }
+    jg.resize(total);
+    for (int i = 0; i < n; ++i) std::copy(rows[i].begin(), rows[i].end(), jg.begin() + ig[i]);
+}
+
+std::vector<char> boundary_edge_mask(const ModelFiles& M) {
+    std::vector<char> nb(M.kuzlov, 0), eb(M.n, 0);
+    for (int v : M.l13d)
+        if (v >= 0 && v < M.kuzlov) nb[v] = 1;
+    for (int e = 0; e < M.n; ++e) eb[e] = nb[M.edges[2 * e]] && nb[M.edges[2 * e + 1]];
+    return eb;
+}
+
+namespace {
+
+// Reference-cube basis of local edge j at p in [-1,1]^3 without the 2/h scale
+// (A2/T_Brick.cpp:741-811): x-edges l(y)l(z), y-edges l(z)l(x), z-edges l(x)l(y).
+void ref_basis(int j, const double p[3], double out[3]) {
+    auto l = [](int b, double x) { return b ? 0.5 * (1 + x) : 0.5 * (1 - x); };
+    const int d = j >> 2, k = j & 3;
+    const int pa = (d + 1) % 3, qa = (d + 2) % 3;
+    out[0] = out[1] = out[2] = 0.0;
+    out[d] = l(k & 1, p[pa]) * l(k >> 1, p[qa]);
+}
+
+}  // namespace
+
+SourceInfo place_line_sources(const ModelFiles& M) {
+    SourceInfo S;
+    bool any = false;
+    for (size_t i = 1; i < M.current.size(); ++i)
+        if (M.current[i] != 0.0) any = true;
+    if (!M.fdirect || !any) return S;
+    if ((int)M.wires.size() < M.npls * 6) throw Error(0x0002, "cannot open sours");
+    const int N = M.n_srs_razb;
+    S.shape.assign(size_t(M.npls) * M.n, 0.0);
+    std::vector<double> lo(size_t(M.kpar) * 3), hi(size_t(M.kpar) * 3);
+    for (int e = 0; e < M.kpar; ++e)
+        for (int a = 0; a < 3; ++a) {
+            double mn = 1e300, mx = -1e300;
+            for (int k = 0; k < 8; ++k) {
+                double v = M.xyz[3 * M.nver[8 * e + k] + a];
+                mn = std::min(mn, v);
+                mx = std::max(mx, v);
+            }
+            lo[3 * e + a] = mn - 1e-3;
+            hi[3 * e + a] = mx + 1e-3;
+        }
+    for (int ip = 0; ip < M.npls; ++ip) {
+        const double* W = M.wires.data() + 6 * ip;
+        double dl[3] = {(W[3] - W[0]) / N, (W[4] - W[1]) / N, (W[5] - W[2]) / N};
+        for (int j = 0; j < N; ++j) {
+            double p[3] = {W[0] + (j + 0.5) * dl[0], W[1] + (j + 0.5) * dl[1], W[2] + (j + 0.5) * dl[2]};
+            int elem = -1;
+            double loc[3] = {0.5, 0.5, 0.5};
+            for (int e = 0; e < M.kpar && elem < 0; ++e) {
+                if (p[0] < lo[3 * e] || p[0] > hi[3 * e] || p[1] < lo[3 * e + 1] || p[1] > hi[3 * e + 1] ||
+                    p[2] < lo[3 * e + 2] || p[2] > hi[3 * e + 2])
+                    continue;
+                const double* x0 = &M.xyz[3 * M.nver[8 * e]];
+                const double* x7 = &M.xyz[3 * M.nver[8 * e + 7]];
+                double c2 = 0, l[3];
+                for (int a = 0; a < 3; ++a) {
+                    double ctr = 0.5 * (x0[a] + x7[a]);
+                    c2 += (ctr - p[a]) * (ctr - p[a]);
+                }
+                // Newton on an axis-aligned brick is exact after one step; a start point already
+                // within the tolerance of CheckInHex (A2/CheckInHex.cpp:191-263) is kept.
+                for (int a = 0; a < 3; ++a) l[a] = (c2 <= 1e-6) ? 0.5 : (p[a] - x0[a]) / (x7[a] - x0[a]);
+                if (l[0] > -0.01 && l[0] < 1.01 && l[1] > -0.01 && l[1] < 1.01 && l[2] > -0.01 && l[2] < 1.01) {
+                    elem = e;
+                    for (int a = 0; a < 3; ++a) loc[a] = l[a];
+                }
+            }
+            if (elem < 0) throw Error(0x0004, "Error in find element for sours");
+            S.elem.push_back(elem);
+            for (int a = 0; a < 3; ++a) { S.loc.push_back(loc[a]); S.pos.push_back(p[a]); }
+            const double ref[3] = {2 * loc[0] - 1, 2 * loc[1] - 1, 2 * loc[2] - 1};
+            for (int jj = 0; jj < 12; ++jj) {
+                const int ii = M.ed[12 * elem + jj];
+                const int n0 = M.edges[2 * ii], n1 = M.edges[2 * ii + 1];
+                double len2 = 0;
+                for (int a = 0; a < 3; ++a) {
+                    double d = M.xyz[3 * n1 + a] - M.xyz[3 * n0 + a];
+                    len2 += d * d;
+                }
+                double b[3];
+                ref_basis(jj, ref, b);
+                S.shape[size_t(ip) * M.n + ii] += (b[0] * dl[0] + b[1] * dl[1] + b[2] * dl[2]) * 2.0 / std::sqrt(len2);
+            }
+        }
+    }
+    return S;
+}
+
+namespace {
+
+// Jacobi-preconditioned CG on the nodal mass matrix of a subdomain.
+void pcg(const std::vector<int64_t>& rp, const std::vector<int>& ci, const std::vector<double>& v,
+         const std::vector<double>& b, std::vector<double>& x) {
+    const int n = int(b.size());
+    x.assign(n, 0.0);
+    std::vector<double> r(b), z(n), p(n), q(n), dinv(n, 1.0);
+    for (int i = 0; i < n; ++i)
+        for (int64_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) dinv[i] = 1.0 / v[k];
+    double bn = 0;
+    for (double t : b) bn += t * t;
+    bn = std::sqrt(bn);
+    if (bn == 0) return;
+    for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+    p = z;
+    double rz = 0;
+    for (int i = 0; i < n; ++i) rz += r[i] * z[i];
+    for (int it = 0; it < 20000; ++it) {
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * p[ci[k]];
+            q[i] = s;
+        }
+        double pq = 0;
+        for (int i = 0; i < n; ++i) pq += p[i] * q[i];
+        const double alpha = rz / pq;
+        double rn = 0;
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            rn += r[i] * r[i];
+        }
+        if (std::sqrt(rn) <= 1e-15 * bn) break;
+        for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+        double rz1 = 0;
+        for (int i = 0; i < n; ++i) rz1 += r[i] * z[i];
+        const double beta = rz1 / rz;
+        rz = rz1;
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+}
+
+// Centre value (kind 1) or curl (kind 0) component c of an element as weights on its 12 edges
+// (A2/T_Brick.cpp:846-900 evaluated at the reference centre).
+void centre_weights(int kind, int c, double hx, double hy, double hz, double w[12]) {
+    for (int j = 0; j < 12; ++j) w[j] = 0;
+    const double h[3] = {hx, hy, hz};
+    if (kind == 1) {
+        for (int k = 0; k < 4; ++k) w[4 * c + k] = 1.0 / (2.0 * h[c]);
+        return;
+    }
+    // curl_c = d E_q / d p - d E_p / d q with (c, p, q) cyclic; E_d built from edges 4d..4d+3
+    // whose transverse bits are ((d+1)%3, (d+2)%3). Derivative of E_d along axis a at the
+    // centre: (1/(h_d h_a)) * sum over the 4 edges of sign(bit along a) (bit 1 -> +, 0 -> -).
+    const int p = (c + 1) % 3, q = (c + 2) % 3;
+    auto add = [&](int d, int a, double s) {
+        const int pa = (d + 1) % 3;
+        for (int k = 0; k < 4; ++k) {
+            const int bit = (a == pa) ? (k & 1) : (k >> 1);
+            w[4 * d + k] += s * (bit ? 1.0 : -1.0) / (h[d] * h[a]);
+        }
+    };
+    add(q, p, 1.0);   // + dE_q/dp
+    add(p, q, -1.0);  // - dE_p/dq
+    // E_d = (2/h_d) * bilinear: derivative factor 2/h_d * (1/h_a) * 1/2 = 1/(h_d h_a)
+}
+
+}  // namespace
+
+ReceiverOps build_receiver_ops(const ModelFiles& M, const std::vector<double>& pts, int kind) {
+    ReceiverOps R;
+    const int nrec = int(pts.size() / 3);
+    R.nrec = nrec;
+    R.rp.assign(size_t(3) * nrec + 1, 0);
+    if (nrec == 0) return R;
+    // receiver elements: first element whose box +-1e-6 holds the point (OutputResultant3d.cpp:516-753)
+    std::vector<int> elem(nrec, -1);
+    for (int e = 0; e < M.kpar; ++e) {
+        const double* x0 = &M.xyz[3 * M.nver[8 * e]];
+        const double* x7 = &M.xyz[3 * M.nver[8 * e + 7]];
+        for (int r = 0; r < nrec; ++r) {
+            if (elem[r] >= 0) continue;
+            const double* p = &pts[3 * r];
+            bool in = true;
+            for (int a = 0; a < 3; ++a)
+                if (p[a] < x0[a] - 1e-6 || p[a] > x7[a] + 1e-6) in = false;
+            if (in) elem[r] = e;
+        }
+    }
+    for (int r = 0; r < nrec; ++r)
+        if (elem[r] < 0) throw Error(0x0004, "There are no elements for receiver(s) N" + std::to_string(r));
+    // trilinear coefficients (A2/T_Brick.cpp:1792-1805)
+    std::vector<double> cfa(size_t(nrec) * 8);
+    for (int r = 0; r < nrec; ++r) {
+        const int e = elem[r];
+        const double* x0 = &M.xyz[3 * M.nver[8 * e]];
+        const double* x7 = &M.xyz[3 * M.nver[8 * e + 7]];
+        double l[3];
+        for (int a = 0; a < 3; ++a) l[a] = (2.0 * pts[3 * r + a] - x0[a] - x7[a]) / (x7[a] - x0[a]);
+        for (int i = 0; i < 8; ++i) {
+            double v = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                const int b = (i >> a) & 1;
+                v *= b ? 0.5 * (1 + l[a]) : 0.5 * (1 - l[a]);
+            }
+            cfa[8 * r + i] = v;
+        }
+    }
+    // groups of receivers sharing a subdomain
+    std::vector<std::pair<int, std::vector<int>>> groups;
+    if (kind == 0) {
+        std::vector<int> all(nrec);
+        for (int r = 0; r < nrec; ++r) all[r] = r;
+        groups.push_back({-1, all});
+    } else {
+        std::map<int, std::vector<int>> bym;
+        for (int r = 0; r < nrec; ++r) bym[M.nvkat[elem[r]]].push_back(r);
+        for (auto& kv : bym) groups.push_back(kv);
+    }
+    std::vector<std::vector<std::pair<int, double>>> rows(size_t(3) * nrec);
+    std::vector<double> acc(M.n, 0.0);
+    std::vector<char> touched(M.n, 0);
+    for (auto& g : groups) {
+        const int mat = g.first;
+        std::vector<char> in_el(M.kpar, 0), in_nd(M.kuzlov, 0);
+        auto ok = [&](int e) { return mat < 0 || M.nvkat[e] == mat; };
+        for (int r : g.second) {
+            const int e = elem[r];
+            if (!ok(e)) continue;
+            in_el[e] = 1;
+            for (int k = 0; k < 8; ++k) in_nd[M.nver[8 * e + k]] = 1;
+        }
+        for (int lev = 0; lev < M.n_neib; ++lev) {
+            std::vector<int> add;
+            for (int e = 0; e < M.kpar; ++e) {
+                if (in_el[e] || !ok(e)) continue;
+                for (int k = 0; k < 8; ++k)
+                    if (in_nd[M.nver[8 * e + k]]) { add.push_back(e); break; }
+            }
+            for (int e : add) {
+                in_el[e] = 1;
+                for (int k = 0; k < 8; ++k) in_nd[M.nver[8 * e + k]] = 1;
+            }
+        }
+        std::vector<int> els, loc(M.kuzlov, -1);
+        for (int e = 0; e < M.kpar; ++e)
+            if (in_el[e]) els.push_back(e);
+        int nl = 0;
+        for (int v = 0; v < M.kuzlov; ++v)
+            if (in_nd[v]) loc[v] = nl++;
+        // nodal trilinear mass matrix (A2/Hex_Local_Matrix.cpp:87-169) in CSR
+        std::vector<std::map<int, double>> Mm(nl);
+        std::vector<double> vol(els.size());
+        for (size_t ei = 0; ei < els.size(); ++ei) {
+            const int e = els[ei];
+            const double* x0 = &M.xyz[3 * M.nver[8 * e]];
+            const double* x7 = &M.xyz[3 * M.nver[8 * e + 7]];
+            vol[ei] = (x7[0] - x0[0]) * (x7[1] - x0[1]) * (x7[2] - x0[2]);
+            for (int i = 0; i < 8; ++i)
+                for (int j = 0; j < 8; ++j) {
+                    const int nd = __builtin_popcount(unsigned(i ^ j));
+                    const double f = nd == 0 ? 1.0 / 27 : nd == 1 ? 1.0 / 54 : nd == 2 ? 1.0 / 108 : 1.0 / 216;
+                    Mm[loc[M.nver[8 * e + i]]][loc[M.nver[8 * e + j]]] += vol[ei] * f;
+                }
+        }
+        std::vector<int64_t> rp(nl + 1, 0);
+        std::vector<int> ci;
+        std::vector<double> cv;
+        for (int i = 0; i < nl; ++i) {
+            for (auto& kv : Mm[i]) { ci.push_back(kv.first); cv.push_back(kv.second); }
+            rp[i + 1] = int64_t(ci.size());
+        }
+        for (int r : g.second) {
+            if (!ok(elem[r])) continue;
+            std::vector<double> rhs(nl, 0.0), y;
+            for (int k = 0; k < 8; ++k) rhs[loc[M.nver[8 * elem[r] + k]]] += cfa[8 * r + k];
+            pcg(rp, ci, cv, rhs, y);
+            for (int c = 0; c < 3; ++c) {
+                std::vector<int> tl;
+                for (size_t ei = 0; ei < els.size(); ++ei) {
+                    const int e = els[ei];
+                    double g8 = 0;
+                    for (int k = 0; k < 8; ++k) g8 += y[loc[M.nver[8 * e + k]]];
+                    g8 *= vol[ei] / 8.0;
+                    if (g8 == 0.0) continue;
+                    const double* x0 = &M.xyz[3 * M.nver[8 * e]];
+                    const double* x7 = &M.xyz[3 * M.nver[8 * e + 7]];
+                    double w[12];
+                    centre_weights(kind, c, x7[0] - x0[0], x7[1] - x0[1], x7[2] - x0[2], w);
+                    for (int j = 0; j < 12; ++j) {
+                        if (w[j] == 0.0) continue;
+                        const int ii = M.ed[12 * e + j];
+                        if (!touched[ii]) { touched[ii] = 1; tl.push_back(ii); }
+                        acc[ii] += g8 * w[j];
+                    }
+                }
+                std::sort(tl.begin(), tl.end());
+                auto& row = rows[size_t(3) * r + c];
+                for (int ii : tl) {
+                    row.push_back({ii, acc[ii]});
+                    acc[ii] = 0.0;
+                    touched[ii] = 0;
+                }
+            }
+        }
+    }
+    for (size_t k = 0; k < rows.size(); ++k) R.rp[k + 1] = R.rp[k] + int64_t(rows[k].size());
+    R.idx.resize(R.rp.back());
+    R.w.resize(R.rp.back());
+    for (size_t k = 0; k < rows.size(); ++k)
+        for (size_t j = 0; j < rows[k].size(); ++j) {
+            R.idx[R.rp[k] + j] = rows[k][j].first;
+            R.w[R.rp[k] + j] = rows[k][j].second;
+        }
+    return R;
+}
+
+static int find_time_layer(const std::vector<double>& t, double val) {
+    if (std::fabs(val) < 1e-14 && std::fabs(t[0]) < 1e-14) return 0;
+    for (size_t it = 1; it < t.size(); ++it)
+        if (std::fabs(1.0 - t[it] / val) < 1e-6) return int(it);
+    return -1;
+}
+
+std::vector<Decade> decade_plan(const std::vector<double>& t) {
+    const int nt = int(t.size());
+    if (nt < 3) throw Error(0x0004, "time grid needs at least 3 layers");
+    std::vector<int> dec{0};
+    double htpre = t[1] - t[0];
+    for (int i = 2; i < nt; ++i) {
+        const double htcur = t[i] - t[i - 1];
+        if (std::fabs(1.0 - htpre / htcur) > 1e-3) { dec.push_back(i); htpre = htcur; }
+    }
+    dec.push_back(nt);
+    const int nd = int(dec.size()) - 1;
+    std::vector<Decade> plan;
+    for (int k = 0; k < nd; ++k) {
+        Decade D{dec[k], dec[k + 1], -1, -1, dec[k + 1]};
+        if (k) {
+            if (dec[k] + 1 >= nt) throw Error(0x0004, "decade of a single layer at the end of the grid");
+            const double h = t[dec[k] + 1] - t[dec[k]];
+            D.npre1 = find_time_layer(t, t[dec[k]] - 1 * h);
+            D.npre2 = find_time_layer(t, t[dec[k]] - 2 * h);
+            if (D.npre1 < 0 || D.npre2 < 0) throw Error(0x0004, "Error in npre1 or npre2");
+        }
+        if (k < nd - 1) {
+            const double h = t[dec[k + 1] + 1] - t[dec[k + 1]];
+            D.nlst = find_time_layer(t, t[dec[k + 1]] - 2 * h);
+        }
+        plan.push_back(D);
+    }
+    return plan;
+}
+
+void print_window(const ModelFiles& M, int& first, int& last) {
+    const std::vector<double>& t = M.times;
+    const int nt = int(t.size());
+    first = last = 2;
+    for (int i = 0; i < nt; ++i)
+        if (t[i] >= M.win_left) { first = i; break; }
+    if (first > nt - 1) first = nt - 1;
+    for (int i = 0; i < nt; ++i)
+        if (t[i] >= M.win_right - 1e-6) { last = i; break; }
+    if (M.win_right > t[nt - 1] - 1e-6) last = nt - 1;
+}
+
+void diff_series(const std::vector<double>& t, const double* F, double* out, int first, int last, int stride,
+                 const std::vector<int>& vpre1, const std::vector<int>& vpre2, int scheme) {
+    auto row = [&](int j) -> int64_t {
+        if (j < first || j > last) throw Error(0x0004, "time differentiation reaches outside the print window");
+        return int64_t(j - first) * stride;
+    };
+    auto comb = [&](int jd, int j0, int j1, int j2, double a0, double a1, double a2) {
+        const int64_t o = row(jd), o0 = row(j0), o1 = row(j1), o2 = row(j2);
+        for (int k = 0; k < stride; ++k) out[o + k] = F[o0 + k] * a0 + F[o1 + k] * a1 + F[o2 + k] * a2;
+    };
+    if (last - first < 2) throw Error(0x0004, "print window shorter than 3 layers");
+    int j = first, j1 = first + 1, j2 = first + 2;
+    double dt = t[j2] - t[j], dt1 = t[j1] - t[j], dt0 = t[j2] - t[j1];
+    comb(j, j, j1, j2, -(dt + dt1) / (dt1 * dt), dt / (dt1 * dt0), -dt1 / (dt * dt0));
+    while (true) {
+        if (scheme == 2) {
+            const double m0 = -1.0 / (t[j1] - t[j]), m1 = 1.0 / (t[j1] - t[j]);
+            comb(j1, j, j1, j2, m0, m1, 0.0);
+        } else {
+            comb(j1, j, j1, j2, -dt0 / (dt1 * dt), -(dt1 - dt0) / (dt1 * dt0), dt1 / (dt * dt0));
+        }
+        if (j2 == last) break;
+        if (scheme == 2) {
+            ++j2;
+            j = vpre2[j2];
+            j1 = vpre1[j2];
+        } else {
+            j = j1;
+            ++j2;
+            j1 = vpre1[j2];
+        }
+        dt = t[j2] - t[j];
+        dt1 = t[j1] - t[j];
+        dt0 = t[j2] - t[j1];
+    }
+    if (scheme == 2) {
+        const double m1 = -1.0 / (t[j2] - t[j1]), m2 = 1.0 / (t[j2] - t[j1]);
+        comb(j2, j, j1, j2, 0.0, m1, m2);
+    } else {
+        comb(j2, j, j1, j2, dt0 / (dt1 * dt), -dt / (dt1 * dt0), (dt + dt0) / (dt * dt0));
+    }
+}
+
+void write_tables(const std::string& dir, const ModelFiles& M, int first, int last, const double* eds,
+                  const double* bfield, const double* efield) {
+    const int nrB = int(M.recB.size() / 3), nrE = int(M.recE.size() / 3);
+    std::vector<int> igB(M.npls + 1, 0), igE(M.npls + 1, 0);
+    for (int i = 0; i < M.npls; ++i) {
+        igB[i + 1] = igB[i] + M.recB_per_src[i];
+        igE[i + 1] = igE[i] + M.recE_per_src[i];
+    }
+    struct Tab {
+        const char* stem;
+        const std::vector<double>* rec;
+        const std::vector<int>* ig;
+        const double* arr;
+        double scale;
+        const char* head;
+    };
+    const Tab tabs[3] = {
+        {"edsall_anom", &M.recB, &igB, eds, -1000.0,
+         "      t (ms)         Emfx_anom (mV)     Emfy_anom (mV)     Emfz_anom (mV)\n"},
+        {"ball_anom", &M.recB, &igB, bfield, 1000.0,
+         "      t (ms)        Bx_anom (mTl)     By_anom (mTl)     Bz_anom (mTl)\n"},
+        {"eall_anom", &M.recE, &igE, efield, 1000.0,
+         "      t (ms)         Ex_anom n (mV)     Ey_anom a (mV)     Ez_anom s (mV)\n"},
+    };
+    for (int ip = 0; ip < M.npls; ++ip)
+        for (const Tab& T : tabs) {
+            const int nr = int(T.rec->size() / 3);
+            const std::string path = join_path(dir, std::string(T.stem) + "." + std::to_string(ip + 1));
+            FILE* f = std::fopen(path.c_str(), "w");
+            if (!f) throw Error(0x0002, "cannot write " + path);
+            for (int i = 0; i < nr; ++i) {
+                if (i < (*T.ig)[ip] || i >= (*T.ig)[ip + 1]) continue;
+                const double* p = &(*T.rec)[3 * i];
+                std::fprintf(f, "%d %d\n", int(p[0]), int(p[1]));
+                std::fprintf(f, "%.16e %.16e %.16e\n", p[0], p[1], p[2]);
+                std::fprintf(f, "1\n1 1 1 1\n1\n");
+                std::fputs(T.head, f);
+                for (int j = first; j <= last; ++j) {
+                    double v[3] = {0, 0, 0};
+                    if (T.arr) {
+                        const int64_t o = ((int64_t(j - first) * M.npls + ip) * nr + i) * 3;
+                        for (int c = 0; c < 3; ++c) v[c] = T.arr[o + c] * T.scale;
+                    }
+                    std::fprintf(f, "%.16e\t%.16e\t%.16e\t%.16e\n", M.times[j] * 1000, v[0], v[1], v[2]);
+                }
+            }
+            if (nr == 0) std::fprintf(f, " \n");
+            std::fclose(f);
+        }
+    (void)nrB;
+    (void)nrE;
+}
+
+}  // namespace emtd

@@ -1,0 +1,212 @@
+// Input-file readers of the calculation module (formats of the reference's working directory).
+#include "model_io.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "common.hpp"
+
+namespace emtd {
+
+static const double kMu0 = 1.25663706143591729e-6;  // A2/vfem_const.h:21
+
+std::string join_path(const std::string& dir, const std::string& name) {
+    if (dir.empty() || dir == ".") return name;
+    return dir.back() == '/' ? dir + name : dir + "/" + name;
+}
+
+bool file_exists(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fclose(f);
+    return true;
+}
+
+namespace {
+
+template <typename T>
+std::vector<T> read_bin(const std::string& path, size_t count) {
+    std::vector<T> v(count);
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw Error(0x0002, "cannot open " + path);
+    size_t got = count ? std::fread(v.data(), sizeof(T), count, f) : 0;
+    std::fclose(f);
+    if (got != count) throw Error(0x0040, "cannot read binary file " + path);
+    return v;
+}
+
+std::string read_text(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw Error(0x0002, "cannot open " + path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+}
+
+bool read_flag(const std::string& path, double& v) {
+    std::ifstream in(path);
+    if (!in) return false;
+    double x;
+    if (in >> x) { v = x; return true; }
+    return false;
+}
+
+void read_material_table(const std::string& path, std::vector<double>& a, std::vector<double>& b, int& nmat) {
+    std::ifstream in(path);
+    if (!in) throw Error(0x0002, "cannot open " + path);
+    std::vector<int> idx;
+    std::vector<double> x, y;
+    int i;
+    double p, q;
+    while (in >> i >> p >> q) { idx.push_back(i); x.push_back(p); y.push_back(q); }
+    int mx = 0;
+    for (int k : idx) mx = std::max(mx, k);
+    if (nmat == 0) nmat = mx;
+    a.assign(nmat, 0.0);
+    b.assign(nmat, 0.0);
+    for (size_t k = 0; k < idx.size(); ++k)
+        if (idx[k] >= 1 && idx[k] <= nmat) { a[idx[k] - 1] = x[k]; b[idx[k] - 1] = y[k]; }
+}
+
+std::vector<double> read_points(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw Error(0x0002, "cannot open " + path);
+    long n = 0;
+    in >> n;
+    std::vector<double> v(size_t(n) * 3);
+    for (auto& x : v)
+        if (!(in >> x)) throw Error(0x0040, "bad receiver file " + path);
+    return v;
+}
+
+}  // namespace
+
+ModelFiles read_model_files(const std::string& dir, int formulation) {
+    ModelFiles M;
+    auto P = [&](const char* name) { return join_path(dir, name); };
+    {  // inftry.dat (A2/in_out.cpp:474-508)
+        std::istringstream ss(read_text(P("inftry.dat")));
+        std::string tok;
+        while (ss >> tok) {
+            if (tok == "KUZLOV=") ss >> M.kuzlov;
+            else if (tok == "KPAR=") ss >> M.kpar;
+            else if (tok == "KT1=") ss >> M.kt1;
+        }
+    }
+    if (M.kuzlov <= 0 || M.kpar <= 0) throw Error(0x0040, "bad inftry.dat");
+    M.xyz = read_bin<double>(P("xyz.dat"), size_t(M.kuzlov) * 3);
+    {
+        std::vector<int> nv = read_bin<int>(P("nver.dat"), size_t(M.kpar) * 14);
+        M.nver.resize(size_t(M.kpar) * 8);
+        for (int e = 0; e < M.kpar; ++e)
+            for (int j = 0; j < 8; ++j) M.nver[8 * e + j] = nv[14 * e + j] - 1;
+    }
+    M.nvkat = read_bin<int>(P("nvkat.dat"), M.kpar);
+    for (int& v : M.nvkat) v -= 1;
+    std::string l13 = file_exists(P("l13d.dat")) ? P("l13d.dat") : P("L13d.dat");
+    M.l13d = read_bin<int>(l13, M.kt1);
+    for (int& v : M.l13d) v -= 1;
+    {  // tsize3d_.dat: "0 n" (A2/time_approx.cpp:254-262)
+        std::istringstream ss(read_text(P("tsize3d_.dat")));
+        int j;
+        ss >> j >> M.n;
+    }
+    M.edges = read_bin<int>(P("nodesforedges.dat"), size_t(M.n) * 2);
+    for (int& v : M.edges) v -= 1;
+    {
+        std::vector<int> e25 = read_bin<int>(P("edges.dat"), size_t(M.kpar) * 25);
+        M.ed.resize(size_t(M.kpar) * 12);
+        for (int e = 0; e < M.kpar; ++e)
+            for (int j = 0; j < 12; ++j) M.ed[12 * e + j] = e25[25 * e + j] - 1;
+    }
+    read_material_table(P("sig3d"), M.sigma, M.sigma0, M.nmat);
+    {
+        std::vector<double> mur, mur0;
+        read_material_table(P("mu3d"), mur, mur0, M.nmat);
+        M.mu.resize(M.nmat);
+        for (int i = 0; i < M.nmat; ++i) M.mu[i] = mur[i] * kMu0;
+    }
+    {  // infite.0 (A2/vec_prep_data.cpp:834-853)
+        std::string txt = read_text(P("infite.0"));
+        size_t pos = txt.find('=');
+        if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
+        int ntime = std::atoi(txt.c_str() + pos + 1);
+        for (int k = 0; k < 4; ++k) {
+            pos = txt.find('\n', pos);
+            if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
+            ++pos;
+        }
+        const char* c = txt.c_str() + pos;
+        M.times.resize(ntime);
+        for (int i = 0; i < ntime; ++i) {
+            char* end;
+            M.times[i] = std::strtod(c, &end);
+            if (end == c) throw Error(0x0040, "bad time value in infite.0");
+            c = std::strchr(end, ';');
+            if (!c) { if (i + 1 < ntime) throw Error(0x0040, "truncated infite.0"); break; }
+            ++c;
+        }
+    }
+    M.recB = read_points(P("xyzVectorB"));
+    M.recE = read_points(P("xyzVectorE"));
+    {  // group, recvsb, recvse (A2/TaskSP.cpp:467-535)
+        std::ifstream in(P("group"));
+        if (!in) throw Error(0x0002, "cannot open group");
+        int ngrp = 0, j, p1, p2;
+        in >> ngrp;
+        for (int g = 0; g < ngrp; ++g) {
+            in >> j >> p1 >> p2;
+            M.npls += p2 - p1 + 1;
+        }
+        auto counts = [&](const char* name) {
+            std::ifstream f(P(name));
+            if (!f) throw Error(0x0002, std::string("cannot open ") + name);
+            std::vector<int> v(M.npls);
+            for (int& x : v) f >> x;
+            return v;
+        };
+        M.recB_per_src = counts("recvsb");
+        M.recE_per_src = counts("recvse");
+    }
+    double v = 0;
+    if (read_flag(P("fdirect"), v)) M.fdirect = int(v);
+    if (read_flag(P("nSrsRazb"), v)) M.n_srs_razb = int(v);
+    if (read_flag(P("n_neib"), v)) M.n_neib = int(v);
+    if (M.fdirect) {
+        std::ifstream in(P("currentfunction"));
+        if (!in) throw Error(0x0002, "cannot open currentfunction");
+        M.current.resize(M.times.size());
+        double t;
+        for (auto& cval : M.current) in >> t >> cval;
+        std::ifstream s(P("sours"));
+        if (s) {
+            M.wires.resize(size_t(M.npls) * 6);
+            for (auto& x : M.wires) s >> x;
+        }
+    }
+    {
+        std::ifstream in(P("timeintervalforprint"));
+        if (!in) throw Error(0x0002, "cannot open timeintervalforprint");
+        in >> M.win_left >> M.win_right;
+    }
+    {
+        std::ifstream in(P("emtd_variant"));
+        std::string f;
+        int sch;
+        if (in >> f >> sch) {
+            M.variant_form = (f == "E" || f == "e") ? 1 : 0;
+            M.variant_scheme = sch;
+        }
+    }
+    const int form = formulation >= 0 ? formulation : (M.variant_form >= 0 ? M.variant_form : 0);
+    if (form == 0)
+        M.a0 = read_bin<double>(P("a0.dat"), size_t(M.npls) * M.kuzlov * 3);
+    else
+        M.v3a = read_bin<double>(P("v3a.dat"), size_t(M.npls) * M.kuzlov);
+    return M;
+}
+
+}  // namespace emtd

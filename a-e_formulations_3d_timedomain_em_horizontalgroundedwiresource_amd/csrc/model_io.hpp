@@ -1,0 +1,40 @@
+// Readers of the calculation module's input files and writers of its output tables.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace emtd {
+
+struct ModelFiles {
+    int kuzlov = 0, kpar = 0, kt1 = 0, n = 0, nmat = 0;
+    std::vector<double> xyz;                 // kuzlov*3
+    std::vector<int> nver;                   // kpar*8 (0-based)
+    std::vector<int> nvkat;                  // kpar (0-based)
+    std::vector<int> l13d;                   // kt1 (0-based)
+    std::vector<int> edges;                  // n*2 (0-based)
+    std::vector<int> ed;                     // kpar*12 (0-based)
+    std::vector<double> sigma, sigma0, mu;   // per material (mu absolute)
+    std::vector<double> times;
+    std::vector<double> recB, recE;          // points*3
+    int npls = 0;
+    std::vector<int> recB_per_src, recE_per_src;
+    int fdirect = 0;
+    std::vector<double> current;             // per time layer (empty when absent)
+    std::vector<double> wires;               // npls*6: A(x,y,z) B(x,y,z)
+    int n_srs_razb = 500;
+    int n_neib = 5;
+    double win_left = 0, win_right = 0;
+    std::vector<double> a0;                  // npls*kuzlov*3 (A-formulation)
+    std::vector<double> v3a;                 // npls*kuzlov (E-formulation)
+    int variant_form = -1, variant_scheme = -1;  // from "emtd_variant" when present
+};
+
+// Reads every input of the module from directory dir (A2/vec_prep_data.cpp:83-257, 834-853,
+// 982-1045; A2/time_approx.cpp:243-287; A2/TaskSP.cpp:467-535; A2/give_out_vec_loop.cpp:46-73).
+ModelFiles read_model_files(const std::string& dir, int formulation);
+
+std::string join_path(const std::string& dir, const std::string& name);
+bool file_exists(const std::string& path);
+
+}  // namespace emtd

@@ -1,0 +1,90 @@
+// Per-time-step kernels of the implicit scheme (everything except the triangular sweeps).
+//
+// RHS of one step (A2/time_approx.cpp:1018-1081, 3-point: A3/time_approx.cpp:943-945):
+//   b = C (mult1*u_j1 + mult2*u_j2) + scale * s,   b[Dirichlet] = 0
+// with C the sigma-mass matrix in permuted full CSR (only same-direction couplings are
+// stored), s the line-source shape and scale = I(t) (A) or -dI/dt (E). Dirichlet DOFs are the
+// trailing permuted indices [nn, n), so they are simply not written (the solve ignores them).
+//
+// Receivers (A2/OutputResultant3d.cpp:421-515): each receiver component is a fixed linear
+// functional of the edge field (centre values -> nodal L2 projection -> trilinear
+// interpolation); its weights are precomputed, so one step is a small sparse mat-vec.
+#include <hip/hip_runtime.h>
+
+#include "step.hpp"
+
+namespace emtd {
+namespace {
+
+__global__ void __launch_bounds__(256) k_rhs(int nn, int64_t n, int npls, const int64_t* __restrict__ rp,
+                                             const int* __restrict__ ci, const double* __restrict__ cv,
+                                             const double* __restrict__ u1, const double* __restrict__ u2,
+                                             double m1, double m2, const double* __restrict__ src, double scale,
+                                             double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int64_t k0 = rp[i], k1 = rp[i + 1];
+    for (int r = 0; r < npls; ++r) {
+        const double* a1 = u1 + r * n;
+        const double* a2 = u2 + r * n;
+        double acc = 0.0;
+        if (m2 != 0.0) {
+            for (int64_t k = k0; k < k1; ++k) {
+                const int j = ci[k];
+                acc += cv[k] * (a1[j] * m1 + a2[j] * m2);
+            }
+        } else {
+            for (int64_t k = k0; k < k1; ++k) acc += cv[k] * (a1[ci[k]] * m1);
+        }
+        if (src) acc += src[i + r * n] * scale;
+        b[i + r * n] = acc;
+    }
+}
+
+__global__ void k_zero_tail(double* __restrict__ x, int nn, int64_t n, int npls) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t cnt = n - nn;
+    if (k >= cnt * npls) return;
+    const int r = int(k / cnt);
+    x[r * n + nn + (k % cnt)] = 0.0;
+}
+
+// out[r*nrows + row] = sum_k w[k] u[idx[k] + r*n]; one wavefront per (row, source).
+__global__ void k_receivers(int nrows, int npls, const int64_t* __restrict__ rp, const int* __restrict__ idx,
+                            const double* __restrict__ w, const double* __restrict__ u, int64_t n,
+                            double* __restrict__ out) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nrows * npls) return;
+    const int row = wave % nrows, r = wave / nrows;
+    double acc = 0.0;
+    for (int64_t k = rp[row] + lane; k < rp[row + 1]; k += 64) acc += w[k] * u[idx[k] + r * n];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) out[int64_t(r) * nrows + row] = acc;
+}
+
+}  // namespace
+
+void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
+                const double* u1, const double* u2, double m1, double m2, const double* src, double scale,
+                double* b) {
+    if (nn <= 0) return;
+    hipLaunchKernelGGL(k_rhs, dim3((nn + 255) / 256), dim3(256), 0, s, nn, n, npls, rp, ci, cv, u1, u2, m1, m2, src,
+                       scale, b);
+}
+
+void launch_zero_tail(hipStream_t s, double* x, int nn, int64_t n, int npls) {
+    const int64_t cnt = (n - nn) * npls;
+    if (cnt <= 0) return;
+    hipLaunchKernelGGL(k_zero_tail, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, s, x, nn, n, npls);
+}
+
+void launch_receivers(hipStream_t s, int nrows, int npls, const int64_t* rp, const int* idx, const double* w,
+                      const double* u, int64_t n, double* out) {
+    const int waves = nrows * npls;
+    if (waves <= 0) return;
+    hipLaunchKernelGGL(k_receivers, dim3((waves * 64 + 255) / 256), dim3(256), 0, s, nrows, npls, rp, idx, w, u, n,
+                       out);
+}
+
+}  // namespace emtd

@@ -1,0 +1,17 @@
+// Launch wrappers of the per-step kernels (step.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace emtd {
+
+void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
+                const double* u1, const double* u2, double m1, double m2, const double* src, double scale,
+                double* b);
+void launch_zero_tail(hipStream_t s, double* x, int nn, int64_t n, int npls);
+void launch_receivers(hipStream_t s, int nrows, int npls, const int64_t* rp, const int* idx, const double* w,
+                      const double* u, int64_t n, double* out);
+
+}  // namespace emtd

@@ -1,0 +1,394 @@
+"""Synthetic model generator writing the input files of the implicit 3D time-domain step.
+
+The files written here are exactly the ones ``CalcNonstat3DLine`` reads from its working
+directory (reference: ``A2/vec_prep_data.cpp:83-257`` for the mesh/materials/receivers,
+``A2/vec_prep_data.cpp:834-853`` for ``infite.0``, ``A2/vec_prep_data.cpp:982-1045`` for the
+current function and the line source, ``A2/time_approx.cpp:243-287`` for ``tsize3d_.dat``,
+``nodesforedges.dat`` and ``edges.dat``, ``A2/TaskSP.cpp:467-535`` for ``group``/``recvsb``/
+``recvse``, ``A2/give_out_vec_loop.cpp:46-73`` for ``timeintervalforprint``).
+
+The mesh layout follows the reference mesh writer ``M/RegularMeshBuilder``:
+
+* nodes numbered x fastest, then y, then z (``ReadWrite.cpp:287-320``);
+* ``nver.dat``: 14 int32 per element, the 8 vertices in (x, y, z) bit order, then zeros
+  (``ReadWrite.cpp:329-388``);
+* edges numbered by the lexicographic order of their (first node, second node) pairs
+  (a ``std::set<pair<int,int>>`` in ``Mesh3D.cpp:130-172``), i.e. node by node, and at every
+  node in the order x-edge, y-edge, z-edge;
+* ``edges.dat``: 12 edge numbers per element in the local order of ``Mesh3D.cpp:198-225``,
+  then 12 zeros and a 1 (``ReadWrite.cpp:770-800``);
+* ``l13d.dat``: the 1-based numbers of all nodes on the outer box (``ReadWrite.cpp:710-740``).
+
+All integers in the binary files are 1-based, as the reference readers subtract 1.
+
+The physical content (grading, layers, blocks, random fields, the DC initial field) is
+synthetic and seeded; the grid and time-mesh conventions are the reference's.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MU_0 = 1.25663706143591729e-6
+
+
+# ----------------------------------------------------------------------------- 1D meshes
+def graded_axis(ncells: int, h0: float, q: float, nneg: int | None = None) -> np.ndarray:
+    """Node coordinates of a 1D axis: cells of size h0*q**k growing away from 0.
+
+    ``nneg`` cells lie on the negative side of 0, the rest on the positive side
+    (default: half and half).
+    """
+    if nneg is None:
+        nneg = ncells // 2
+    npos = ncells - nneg
+    neg = -np.cumsum(h0 * q ** np.arange(nneg))[::-1]
+    pos = np.cumsum(h0 * q ** np.arange(npos))
+    return np.concatenate([neg, [0.0], pos])
+
+
+# ----------------------------------------------------------------------------- model spec
+@dataclass
+class Block:
+    x0: float
+    x1: float
+    y0: float
+    y1: float
+    z0: float
+    z1: float
+    sigma: float
+
+
+@dataclass
+class ModelSpec:
+    """A tensor-grid model. z grows upward; z=0 is the earth surface."""
+    xs: np.ndarray
+    ys: np.ndarray
+    zs: np.ndarray
+    sigma_air: float = 1e-8
+    layer_tops: list = field(default_factory=list)      # depths of layer tops (>=0), first is 0
+    layer_sigma: list = field(default_factory=list)
+    blocks: list = field(default_factory=list)
+    voxel_sigma: np.ndarray | None = None                # optional (V,V,V) field over the earth cells
+    formulation: str = "A"                               # "A" or "E"
+    scheme: int = 2                                      # 2 or 3
+    times: np.ndarray | None = None
+    current: np.ndarray | None = None                    # I(t_i); step-off when None
+    wire: tuple = ((-500.0, 0.0, 0.0), (500.0, 0.0, 0.0))
+    receivers: np.ndarray | None = None                  # (nrec, 3)
+    n_neib: int = 3
+    n_srs_razb: int = 500
+    print_window: tuple | None = None                    # (left, right) seconds
+
+
+def time_grid(t_end: float, steps_per_decade: int, ndec: int) -> np.ndarray:
+    """Piecewise-uniform grid from 0: ``ndec`` decades of ``steps_per_decade`` steps, dt doubling."""
+    dt0 = t_end / (steps_per_decade * (2.0 ** ndec - 1.0))
+    t = [0.0]
+    for d in range(ndec):
+        for _ in range(steps_per_decade):
+            t.append(t[-1] + dt0 * 2.0 ** d)
+    return np.asarray(t)
+
+
+def quantize_times(t: np.ndarray) -> np.ndarray:
+    """Round to the 8 significant digits ``infite.0`` holds (``TimeStep.cpp:144-172``)."""
+    return np.array([float(f"{v:.7e}") for v in t])
+
+
+# ----------------------------------------------------------------------------- topology
+def grid_topology(nxn: int, nyn: int, nzn: int):
+    """Edges, element→edge and element→node tables of a tensor grid with the given node counts.
+
+    Returns (edges[n,2] 0-based node ids, ed[kpar,12] 0-based edge ids, nver[kpar,8] 0-based).
+    """
+    nxy = nxn * nyn
+    kuz = nxy * nzn
+    node = np.arange(kuz)
+    ix = node % nxn
+    iy = (node // nxn) % nyn
+    iz = node // nxy
+    hasx = ix < nxn - 1
+    hasy = iy < nyn - 1
+    hasz = iz < nzn - 1
+    cnt = hasx.astype(np.int64) + hasy + hasz
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    ex = np.where(hasx, first, -1)
+    ey = np.where(hasy, first + hasx, -1)
+    ez = np.where(hasz, first + hasx + hasy, -1)
+    n = int(cnt.sum())
+    edges = np.empty((n, 2), dtype=np.int64)
+    m = hasx
+    edges[ex[m], 0] = node[m]
+    edges[ex[m], 1] = node[m] + 1
+    m = hasy
+    edges[ey[m], 0] = node[m]
+    edges[ey[m], 1] = node[m] + nxn
+    m = hasz
+    edges[ez[m], 0] = node[m]
+    edges[ez[m], 1] = node[m] + nxy
+    # elements
+    cx, cy, cz = nxn - 1, nyn - 1, nzn - 1
+    e = np.arange(cx * cy * cz)
+    ex_ = e % cx
+    ey_ = (e // cx) % cy
+    ez_ = e // (cx * cy)
+    n0 = ez_ * nxy + ey_ * nxn + ex_
+    nver = np.stack([n0, n0 + 1, n0 + nxn, n0 + nxn + 1,
+                     n0 + nxy, n0 + nxy + 1, n0 + nxy + nxn, n0 + nxy + nxn + 1], axis=1)
+    ed = np.stack([
+        ex[n0], ex[n0 + nxn], ex[n0 + nxy], ex[n0 + nxy + nxn],          # x-edges (y,z) = 00,10,01,11
+        ey[n0], ey[n0 + nxy], ey[n0 + 1], ey[n0 + nxy + 1],              # y-edges (z,x) = 00,10,01,11
+        ez[n0], ez[n0 + 1], ez[n0 + nxn], ez[n0 + nxn + 1],              # z-edges (x,y) = 00,10,01,11
+    ], axis=1)
+    return edges, ed, nver
+
+
+def boundary_nodes(nxn: int, nyn: int, nzn: int) -> np.ndarray:
+    node = np.arange(nxn * nyn * nzn)
+    ix = node % nxn
+    iy = (node // nxn) % nyn
+    iz = node // (nxn * nyn)
+    m = (ix == 0) | (ix == nxn - 1) | (iy == 0) | (iy == nyn - 1) | (iz == 0) | (iz == nzn - 1)
+    return node[m]
+
+
+# ----------------------------------------------------------------------------- materials
+def cell_materials(spec: ModelSpec):
+    """Material number (1-based) per cell and the (sigma, sigma0) table.
+
+    Material 1 is air; then one material per layer; then one per block; then (optional)
+    one per voxel of the random field.
+    """
+    xs, ys, zs = spec.xs, spec.ys, spec.zs
+    xc = 0.5 * (xs[:-1] + xs[1:])
+    yc = 0.5 * (ys[:-1] + ys[1:])
+    zc = 0.5 * (zs[:-1] + zs[1:])
+    cx, cy, cz = len(xc), len(yc), len(zc)
+    X = np.broadcast_to(xc[None, None, :], (cz, cy, cx))
+    Y = np.broadcast_to(yc[None, :, None], (cz, cy, cx))
+    Z = np.broadcast_to(zc[:, None, None], (cz, cy, cx))
+    mat = np.ones((cz, cy, cx), dtype=np.int64)
+    sig = [spec.sigma_air]
+    sig0 = [spec.sigma_air]
+    depth = -Z
+    earth = Z < 0
+    nl = len(spec.layer_sigma)
+    layer_of = np.zeros_like(mat)
+    for li in range(nl):
+        top = spec.layer_tops[li]
+        layer_of[earth & (depth >= top)] = li
+    for li in range(nl):
+        sig.append(spec.layer_sigma[li])
+        sig0.append(spec.layer_sigma[li])
+    if nl:
+        mat[earth] = 2 + layer_of[earth]
+    nbase = len(sig)
+    for bi, b in enumerate(spec.blocks):
+        m = (X > b.x0) & (X < b.x1) & (Y > b.y0) & (Y < b.y1) & (Z > b.z0) & (Z < b.z1)
+        normal = np.asarray(sig0)[mat[m] - 1]
+        mat[m] = nbase + bi + 1
+        sig.append(b.sigma)
+        sig0.append(float(normal[0]) if normal.size else spec.sigma_air)
+    if spec.voxel_sigma is not None:
+        V = spec.voxel_sigma.shape[0]
+        ez = np.nonzero(zc < 0)[0]
+        ncz = len(ez)
+        ix = np.minimum((np.arange(cx) * V) // cx, V - 1)
+        iy = np.minimum((np.arange(cy) * V) // cy, V - 1)
+        izl = np.minimum((np.arange(ncz) * V) // max(ncz, 1), V - 1)
+        base = len(sig)
+        vox = (izl[:, None, None] * V + iy[None, :, None]) * V + ix[None, None, :]
+        mat[ez[0]:ez[-1] + 1] = base + 1 + vox
+        flat = spec.voxel_sigma.reshape(-1)
+        sig.extend(flat.tolist())
+        sig0.extend(flat.tolist())
+    return mat.reshape(-1), np.asarray(sig), np.asarray(sig0)
+
+
+# ----------------------------------------------------------------------------- DC fields
+def wire_vector_potential(xyz: np.ndarray, a, b, rmin: float) -> np.ndarray:
+    """Whole-space vector potential of a straight 1 A wire from a to b (closed form)."""
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    L = np.linalg.norm(b - a)
+    t = (b - a) / L
+    d = xyz - a
+    s = d @ t
+    perp = d - s[:, None] * t
+    rho = np.maximum(np.linalg.norm(perp, axis=1), rmin)
+    val = MU_0 / (4 * np.pi) * (np.arcsinh(s / rho) - np.arcsinh((s - L) / rho))
+    return val[:, None] * t[None, :]
+
+
+def electrode_potential(xyz: np.ndarray, a, b, sigma: float, rmin: float) -> np.ndarray:
+    """Half-space DC potential of a +1 A electrode at a and -1 A at b."""
+    ra = np.maximum(np.linalg.norm(xyz - np.asarray(a), axis=1), rmin)
+    rb = np.maximum(np.linalg.norm(xyz - np.asarray(b), axis=1), rmin)
+    return (1.0 / ra - 1.0 / rb) / (2 * np.pi * sigma)
+
+
+# ----------------------------------------------------------------------------- writer
+def _w_i32(path, arr):
+    np.ascontiguousarray(arr, dtype="<i4").tofile(path)
+
+
+def _w_f64(path, arr):
+    np.ascontiguousarray(arr, dtype="<f8").tofile(path)
+
+
+def write_model(spec: ModelSpec, out: str) -> dict:
+    """Write every input file of the 3D time-domain step into directory ``out``."""
+    os.makedirs(out, exist_ok=True)
+    xs, ys, zs = spec.xs, spec.ys, spec.zs
+    nxn, nyn, nzn = len(xs), len(ys), len(zs)
+    kuz = nxn * nyn * nzn
+    kpar = (nxn - 1) * (nyn - 1) * (nzn - 1)
+    edges, ed, nver = grid_topology(nxn, nyn, nzn)
+    n = edges.shape[0]
+    bnd = boundary_nodes(nxn, nyn, nzn)
+
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    xyz = np.stack([X.transpose(2, 1, 0).reshape(-1), Y.transpose(2, 1, 0).reshape(-1),
+                    Z.transpose(2, 1, 0).reshape(-1)], axis=1)
+
+    with open(os.path.join(out, "inftry.dat"), "w") as f:
+        f.write(f" ISLAU= 0 INDKU1= 0 INDFPO= 0\nKUZLOV= {kuz}   KPAR= {kpar}    KT1= {len(bnd)}"
+                f"   KTR2= 0   KTR3= 0\n   KT8= 0   KT9= 0\nKISRS1= 0 KISRS2= 0 KISRS3= 0   KBRS= 0\n"
+                f"   KT7= 0   KT10= 0  KTR4= 0  KTSIM= 0\n   KT6= 0")
+    _w_f64(os.path.join(out, "xyz.dat"), xyz)
+    nv14 = np.zeros((kpar, 14), dtype=np.int64)
+    nv14[:, :8] = nver + 1
+    _w_i32(os.path.join(out, "nver.dat"), nv14)
+    mat, sig, sig0 = cell_materials(spec)
+    _w_i32(os.path.join(out, "nvkat.dat"), mat)
+    _w_i32(os.path.join(out, "l13d.dat"), bnd + 1)
+    with open(os.path.join(out, "tsize3d_.dat"), "w") as f:
+        f.write(f"0\n{n}\n")
+    _w_i32(os.path.join(out, "nodesforedges.dat"), edges + 1)
+    ed25 = np.zeros((kpar, 25), dtype=np.int64)
+    ed25[:, :12] = ed + 1
+    ed25[:, 24] = 1
+    _w_i32(os.path.join(out, "edges.dat"), ed25)
+    nmat = len(sig)
+    with open(os.path.join(out, "sig3d"), "w") as f:
+        for i in range(nmat):
+            f.write(f"{i + 1} {sig[i]:.17g} {sig0[i]:.17g}\n")
+    with open(os.path.join(out, "mu3d"), "w") as f:
+        for i in range(nmat):
+            f.write(f"{i + 1} 1 1\n")
+
+    # time mesh, written with the 8 significant digits of TimeDirectSolver
+    t = spec.times if spec.times is not None else time_grid(1e-2, 20, 10)
+    with open(os.path.join(out, "infite.0"), "w") as f:
+        nt = len(t)
+        f.write(f"    ktime=  {nt};  ntstop=  {nt};   kiter=    1;   ntime=    1;   niter=    1;\n")
+        f.write("   kprogm=    1;  kpropi=    1;  kpropt=   -1;  kitrel=  250;              ;\n")
+        f.write("       u0=   0.00000e+000\n")
+        f.write(" T I M E :\n")
+        for i in range(0, nt, 5):
+            f.write("".join(f" {v:.7e};" for v in t[i:i + 5]) + "\n")
+    tq = quantize_times(t)
+    cur = spec.current if spec.current is not None else np.where(np.arange(len(t)) == 0, 1.0, 0.0)
+    with open(os.path.join(out, "currentfunction"), "w") as f:
+        for ti, ci in zip(tq, cur):
+            f.write(f"{ti:.7e} {ci:.17g}\n")
+    (ax, ay, az), (bx, by, bz) = spec.wire
+    with open(os.path.join(out, "sours"), "w") as f:
+        f.write(f"{ax:.17g} {ay:.17g} {az:.17g}\n{bx:.17g} {by:.17g} {bz:.17g}\n")
+    with open(os.path.join(out, "group"), "w") as f:
+        f.write("1\n1 1 1\n")
+    rec = spec.receivers if spec.receivers is not None else default_receivers()
+    for name in ("xyzVectorB", "xyzVectorE"):
+        with open(os.path.join(out, name), "w") as f:
+            f.write(f"{len(rec)}\n")
+            for r in rec:
+                f.write(f"{r[0]:.17g} {r[1]:.17g} {r[2]:.17g}\n")
+    for name in ("recvsb", "recvse"):
+        with open(os.path.join(out, name), "w") as f:
+            f.write(f"{len(rec)}\n")
+    left, right = spec.print_window if spec.print_window else (tq[1], tq[-1])
+    with open(os.path.join(out, "timeintervalforprint"), "w") as f:
+        f.write(f"{left:.17g} {right:.17g}\n")
+    for name, val in (("fdirect", 1), ("n_neib", spec.n_neib), ("nSrsRazb", spec.n_srs_razb),
+                      ("nthreads.txt", 1)):
+        with open(os.path.join(out, name), "w") as f:
+            f.write(f"{val}\n")
+
+    hmin = float(min(np.diff(xs).min(), np.diff(ys).min(), np.diff(zs).min()))
+    if spec.formulation == "A":
+        a0 = wire_vector_potential(xyz, spec.wire[0], spec.wire[1], rmin=0.5 * hmin)
+        _w_f64(os.path.join(out, "a0.dat"), a0)
+    else:
+        s1 = spec.layer_sigma[0] if spec.layer_sigma else 0.01
+        v = electrode_potential(xyz, spec.wire[0], spec.wire[1], s1, rmin=0.5 * hmin)
+        _w_f64(os.path.join(out, "v3a.dat"), v)
+    with open(os.path.join(out, "emtd_variant"), "w") as f:
+        f.write(f"{spec.formulation} {spec.scheme}\n")
+    return dict(n=n, kuzlov=kuz, kpar=kpar, nt=len(t), nmat=nmat)
+
+
+def default_receivers(y: float = 300.0, n: int = 21, half: float = 1000.0) -> np.ndarray:
+    """21 surface receivers on a 2 km profile parallel to the wire."""
+    x = np.linspace(-half, half, n)
+    return np.stack([x, np.full(n, y), np.zeros(n)], axis=1)
+
+
+# ----------------------------------------------------------------------------- configs
+def config_spec(cfg: int, small: tuple | None = None, seed: int | None = None) -> ModelSpec:
+    """The model of a named workload.
+
+    cfg 1: 40x40x30 cells, A, 2-point, 3-layer earth + 1 S/m block (~152k edges).
+    cfg 2: 100x100x60 cells, E, 3-point, 5 random layers + 3 random blocks (~1.84M edges).
+    cfg 3: 80x80x50 cells, A, 2-point, 64^3 log-normal voxel field (~989k edges).
+    ``small`` overrides the cell counts (for tests), keeping the model.
+    """
+    if cfg == 1:
+        cells = (40, 40, 30)
+    elif cfg == 2:
+        cells = (100, 100, 60)
+    elif cfg == 3:
+        cells = (80, 80, 50)
+    else:
+        raise ValueError(cfg)
+    if small is not None:
+        cells = small
+    cx, cy, cz = cells
+    h0, q = 10.0, 1.15
+    xs = graded_axis(cx, h0, q)
+    ys = graded_axis(cy, h0, q)
+    zs = graded_axis(cz, h0, q, nneg=cz // 2)
+    rng = np.random.default_rng(cfg if seed is None else seed)
+    if cfg == 1:
+        spec = ModelSpec(xs, ys, zs, layer_tops=[0.0, 100.0, 150.0], layer_sigma=[0.02, 0.2, 0.05],
+                         blocks=[Block(-100, 100, -100, 100, -200, -150, 1.0)],
+                         formulation="A", scheme=2, times=time_grid(1e-2, 20, 10))
+    elif cfg == 2:
+        tops = [0.0, 50.0, 150.0, 300.0, 600.0]
+        sig = 10 ** rng.uniform(-3, 0, size=5)
+        blocks = []
+        for _ in range(3):
+            cxb, cyb = rng.uniform(-400, 400, size=2)
+            w = rng.uniform(100, 300)
+            d = rng.uniform(50, 300)
+            th = rng.uniform(30, 100)
+            blocks.append(Block(cxb - w / 2, cxb + w / 2, cyb - w / 2, cyb + w / 2, -d - th, -d,
+                                float(10 ** rng.uniform(-0.5, 0.5))))
+        spec = ModelSpec(xs, ys, zs, layer_tops=tops, layer_sigma=sig.tolist(), blocks=blocks,
+                         formulation="E", scheme=3, times=time_grid(1e-1, 25, 12))
+    else:
+        vox = 10 ** rng.normal(-2.0, 0.7, size=(64, 64, 64))
+        spec = ModelSpec(xs, ys, zs, layer_tops=[0.0], layer_sigma=[0.01], voxel_sigma=vox,
+                         formulation="A", scheme=2, times=time_grid(1e-2, 20, 10))
+    if small is not None:
+        # keep the wire and the profile inside a shrunken mesh
+        ext = min(xs[-1], ys[-1])
+        spec.wire = ((-0.4 * ext, 0.0, 0.0), (0.4 * ext, 0.0, 0.0))
+        spec.receivers = default_receivers(y=0.3 * ext, n=7, half=0.6 * ext)
+    if spec.scheme == 3 and spec.print_window is None:
+        # the two backward-Euler start steps output no receiver values (layers 0 and 1)
+        tq = quantize_times(spec.times)
+        spec.print_window = (tq[2], tq[-1])
+    return spec

@@ -1,0 +1,138 @@
+/*
+ * emtd.h — C ABI of the MI355X implicit 3D time-domain EM step (drop-in for CalcNonstat3DLine).
+ *
+ * Plain C: opaque handles, caller-owned host buffers, int return codes (0 = OK). Every entry
+ * point is reentrant per handle; handles are independent of each other. Nothing is global
+ * except the HIP runtime itself.
+ *
+ * Index conventions follow the reference: 0-based arrays in memory (the reference readers
+ * subtract 1 from the 1-based numbers in its files), lower-triangular "RSF" sparse storage
+ * (ig[n+1], jg strictly lower, sorted, unique; ggl values; di diagonal), and column-major
+ * multi-RHS blocks (x[ipls*n + i]).
+ */
+#ifndef EMTD_H
+#define EMTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. Bits 0x1..0x80 keep the meaning of RETCODE_* in A2/TaskSP.h:33-42. */
+#define EMTD_OK          0x0000
+#define EMTD_ERR_NOMEM   0x0001 /* RETCODE_NOMEM */
+#define EMTD_ERR_NOFILE  0x0002 /* RETCODE_NOFILE */
+#define EMTD_ERR_RANGE   0x0004 /* RETCODE_OUTOFRANGE: bad argument or size */
+#define EMTD_ERR_NOTSPD  0x0008 /* RETCODE_SQFROMNEG: non-positive pivot in the factorization */
+#define EMTD_ERR_NOTINIT 0x0020 /* RETCODE_NOTINIT: call order violated */
+#define EMTD_ERR_BADFILE 0x0040 /* RETCODE_BADFILE */
+#define EMTD_ERR_ERROR   0x0080 /* RETCODE_ERROR: HIP runtime or internal error */
+#define EMTD_ERR_STOPPED 0x0200 /* a "stop" file was found (CheckStop, A2/GeoCalculatorSP.cpp:27-39) */
+
+/* Formulation, scheme and precision selectors. */
+#define EMTD_FORM_A   0
+#define EMTD_FORM_E   1
+#define EMTD_SCHEME_2 2 /* backward Euler ("2-point") */
+#define EMTD_SCHEME_3 3 /* BDF2-type ("3-point") */
+#define EMTD_FP64     0 /* factor and solve in FP64 */
+#define EMTD_FP32_IR  1 /* FP32 factor, FP64 residual and refinement, FP64 fallback */
+
+/* Text of the last error on this thread (never NULL). */
+const char* emtd_last_error(void);
+
+/* ---------------------------------------------------------------- mesh description
+ * The brick mesh as the reference holds it after reading its files
+ * (A2/vec_prep_data.cpp:83-257, A2/time_approx.cpp:243-287): nver = 8 vertices per element,
+ * ed = 12 edges per element in the local order of A2/T_Brick.h:203-218, edges = 2 nodes per
+ * edge, nvkat = material per element, sigma/mu per material (mu absolute, H/m). */
+typedef struct emtd_mesh {
+    int kuzlov;          /* nodes */
+    int kpar;            /* elements */
+    int n_edges;         /* edge DOFs */
+    int n_materials;
+    const double* xyz;   /* kuzlov*3 */
+    const int* nver;     /* kpar*8 */
+    const int* ed;       /* kpar*12 */
+    const int* edges;    /* n_edges*2 */
+    const int* nvkat;    /* kpar */
+    const double* sigma; /* n_materials */
+    const double* mu;    /* n_materials */
+} emtd_mesh;
+
+/* ---------------------------------------------------------------- portrait
+ * Replaces T_Portrait::Gen_Portrait (A2/T_Portrait.cpp:45-122): the strictly lower pattern of
+ * the edge-element matrix. Call with jg == NULL to get ig and *nnz; then again with jg of
+ * *nnz entries. */
+int emtd_portrait(const emtd_mesh* mesh, int* ig, int64_t* nnz, int* jg);
+
+/* ---------------------------------------------------------------- assembly (HIP)
+ * Replaces T_Global_SLAE_Vec::Assembling(2) and Assembling(1) (A2/T_Global_SLAE.cpp:76-215,
+ * called at A2/time_approx.cpp:441-470): curl-curl/mu matrix B and sigma-mass matrix C in
+ * lower RSF on the given pattern, computed on the GPU. Host in/out buffers. */
+int emtd_assemble(const emtd_mesh* mesh, const int* ig, const int* jg, int device,
+                  double* b_ggl, double* b_di, double* c_ggl, double* c_di);
+
+/* ---------------------------------------------------------------- direct solver (HIP)
+ * Replaces pardiso_solver (A2/pardiso.h:25-53): factorize() = PARDISO phase 12 on a lower
+ * RSF SPD matrix (A2/pardiso.cpp:69-111), solve_nrhs() = phase 33 (A2/pardiso.cpp:113-120),
+ * stop() = phase -1 (A2/pardiso.cpp:122-126). Supernodal multifrontal Cholesky on the GPU. */
+typedef struct emtd_solver emtd_solver;
+int emtd_solver_create(int device, int precision, emtd_solver** out);
+int emtd_solver_factorize(emtd_solver* s, int n, const int* ig, const int* jg,
+                          const double* ggl, const double* di);
+int emtd_solver_solve_nrhs(emtd_solver* s, int nrhs, const double* pr, double* x);
+int emtd_solver_stop(emtd_solver* s);
+void emtd_solver_destroy(emtd_solver* s);
+
+/* Statistics of the last factorization and solves. */
+typedef struct emtd_solver_stats {
+    int64_t n, n_factored, n_fronts, n_levels;
+    double nnz_l;             /* entries of L */
+    double factor_flops;      /* sum over fronts of the dense partial-factorization flops */
+    double factor_seconds;    /* GPU time of the last numeric factorization */
+    double solve_seconds;     /* GPU time of the last solve_nrhs */
+    int64_t refine_iterations;/* FP32_IR: corrections applied in total */
+    int64_t fallbacks;        /* FP32_IR: FP64 refactorizations triggered */
+    double last_residual;     /* FP32_IR: last relative residual ||b-Ax||/||b|| */
+} emtd_solver_stats;
+int emtd_solver_get_stats(const emtd_solver* s, emtd_solver_stats* st);
+
+/* ---------------------------------------------------------------- the calculation module
+ * Replaces CalcSP (A2/TaskSP.cpp:445-711) with Time_approx_for_vfem's Prepare_To_Schema /
+ * Three_Layers_Schema / FinishCalculation (A2/time_approx.h:105-113): reads the module's input
+ * files from workdir, runs every decade on the GPU, and writes eall_anom.N, edsall_anom.N,
+ * ball_anom.N (A2/give_out_vec_loop.cpp:532-609). formulation/scheme select the reference
+ * variant (A2, A3, E2, E3); a negative value reads them from the file "emtd_variant". */
+typedef struct emtd_calc emtd_calc;
+int emtd_calc_create(const char* workdir, int formulation, int scheme, int precision, int device,
+                     emtd_calc** out);
+int emtd_calc_run(emtd_calc* c);             /* all decades, receivers recorded on device */
+int emtd_calc_write_outputs(emtd_calc* c);   /* Gather_edsall + tables into workdir */
+/* Receiver tables after emtd_calc_run: kind 0 = B receivers (curl of U), 1 = E receivers (U);
+ * out holds (t_last-t_first+1) x npls x nrec x 3 values, undifferentiated. */
+int emtd_calc_receiver_values(const emtd_calc* c, int kind, double* out);
+/* Edge field of the last computed layer, original numbering, n_edges x npls. */
+int emtd_calc_last_field(const emtd_calc* c, double* out);
+
+typedef struct emtd_calc_info {
+    int64_t n_edges, n_elements, n_nodes, npls, ntime, n_decades;
+    int64_t t_first, t_last, n_rec_b, n_rec_e;
+    double setup_seconds;      /* reading + assembly + symbolic analysis */
+    double factor_seconds;     /* all refactorizations */
+    double step_seconds;       /* all per-step work (RHS, solves, receivers) */
+    int64_t steps;
+    double nnz_l, factor_flops;
+    int64_t refine_iterations, fallbacks;
+} emtd_calc_info;
+int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
+void emtd_calc_destroy(emtd_calc* c);
+
+/* One-call drop-in: create + run + write_outputs + destroy (what CalcNonstat3DLine.exe does). */
+int emtd_calc_nonstat_3d_line(const char* workdir, int formulation, int scheme, int precision,
+                              int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMTD_H */

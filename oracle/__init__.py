@@ -1,0 +1,32 @@
+"""CPU oracle for the implicit 3D time-domain step — TEST INFRASTRUCTURE ONLY.
+
+This package is a plain numpy/scipy restatement of the reference module ``CalcNonstat3DLine``
+(A-/E-formulation x 2-/3-point scheme). It exists to check the HIP product path and to serve
+as the timed CPU baseline. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU
+baseline may import it; the product library never does.
+
+Modules:
+
+* ``refio``  — readers of the reference's input files and writers of its output tables;
+* ``fem``    — element matrices, portrait, assembly, Dirichlet conditions;
+* ``calc``   — decades, the time schemes, sources, receivers and time differentiation.
+
+Parity pinning (what checks each function beyond re-typing it):
+
+* element matrices: closed-form entries of a cube and of a general brick taken from the
+  reference's unrolled formulas, brute-force Gauss quadrature of the basis functions,
+  symmetry/definiteness, rank 5 per element and B·G = 0 for the discrete gradient;
+* portrait/assembly: brute-force dense assembly on tiny meshes, entry counts per row;
+* edge numbering: brute-force ``std::set`` ordering of node pairs;
+* decades, npre1/npre2, nlst: hand-built known-answer grids;
+* time schemes: exact discrete solutions of the scalar ODE for backward Euler and BDF2;
+* sources: centre-dipole weights (1/4 on each parallel edge) and the sum rule;
+* receivers: trilinear interpolation of linear fields, L2 projection of constants, exactness
+  of the difference formulas on linear (2-point) and quadratic (3-point) signals;
+* sparse solve: scipy's SuperLU against dense numpy on small systems, and residuals.
+
+Parity unpinned: PARDISO's own rounding and internal refinement (MKL is absent; the solve is
+pinned to residual level), the shared-memory primary/secondary input (``LoadE0FromRZ``),
+and the E-formulation dI/dt stencil for waveforms that are nonzero after t0 (followed
+literally from ``E2/time_approx.cpp:755-775``; no reference output covers it).
+"""

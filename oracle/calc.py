@@ -1,0 +1,544 @@
+"""Time stepping, sources and receivers of the oracle (CPU restatement of ``CalcNonstat3DLine``).
+
+Follows:
+
+* decades: ``GenerateDecInfo``/``FindTimeLayer`` and the decade loop (``A2/TaskSP.cpp:326-398,
+  628-702``);
+* initial state: A from nodal ``a0.dat`` (``A2/time_approx.cpp:608-657``), E from nodal
+  ``v3a.dat`` (``E2/time_approx.cpp:646-678``); edge DOF = tangential value * len/2;
+* 2-point scheme: ``A2/time_approx.cpp:785-1143`` (mult = 1/dt of the decade's first step);
+* 3-point scheme: two backward-Euler steps (``Two_Layers_Schema``, ``A3/time_approx.cpp:575-790``)
+  then BDF2-type steps with mult=(dt+dt0)/(dt*dt0), mult1=dt/(dt1*dt0), mult2=-dt0/(dt*dt1)
+  computed at the decade's first step (``A3/time_approx.cpp:864-945``);
+* system matrix per decade: ``A2/time_approx.cpp:987-996``; RHS ``:1018-1034``; Dirichlet RHS
+  ``:1078-1081``;
+* line source: ``A2/vec_prep_data.cpp:982-1270`` placement, ``A2/time_approx.cpp:1039-1076``
+  (A: I(t) when nonzero) and ``E2/time_approx.cpp:755-775`` (E: -dI/dt, literal stencil);
+* receivers: element search ``A2/OutputResultant3d.cpp:516-753``, subdomains
+  ``A2/Subdomain.cpp:142-417``, centre values ``A2/time_approx.cpp:1481-1518``, projection RHS
+  ``A2/Subdomain.cpp:418-441``, interpolation ``A2/OutputResultant3d.cpp:421-515`` with
+  coefficients of ``A2/T_Brick.cpp:1792-1805``; output layer bookkeeping
+  ``A2/time_approx.cpp:1116-1128``;
+* time differentiation and tables: ``A2/give_out_vec_loop.cpp:240-615``, 3-point variant
+  ``A3/give_out_vec_loop.cpp:313-367``, E-formulation ``E2/give_out_vec_loop.cpp:255-312``.
+
+The sparse direct solve uses scipy's SuperLU in place of MKL PARDISO (absent).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import fem
+from .refio import Model
+
+
+# ----------------------------------------------------------------------------- decades
+def gen_dec_info(times: np.ndarray) -> list:
+    """``GenerateDecInfo``: decade starts where |1 - h_prev/h_cur| > 1e-3."""
+    dec = [0]
+    htpre = times[1] - times[0]
+    for i in range(2, len(times)):
+        htcur = times[i] - times[i - 1]
+        if abs(1.0 - htpre / htcur) > 1e-3:
+            dec.append(i)
+            htpre = htcur
+    dec.append(len(times))
+    return dec
+
+
+def find_time_layer(times: np.ndarray, val: float) -> int:
+    """``FindTimeLayer``: relative match 1e-6 (layer 0 only for val == t0 == 0)."""
+    if abs(val) < 1e-14 and abs(times[0]) < 1e-14:
+        return 0
+    for it in range(1, len(times)):
+        if abs(1.0 - times[it] / val) < 1e-6:
+            return it
+    return -1
+
+
+def decade_plan(times: np.ndarray):
+    """(DecIg, [(tbeg, tend, npre1, npre2, nlst)]) exactly as the decade loop computes them."""
+    dec = gen_dec_info(times)
+    nd = len(dec) - 1
+    plan = []
+    for k in range(nd):
+        npre1 = npre2 = -1
+        if k:
+            h = times[dec[k] + 1] - times[dec[k]]
+            npre1 = find_time_layer(times, times[dec[k]] - 1 * h)
+            npre2 = find_time_layer(times, times[dec[k]] - 2 * h)
+            if npre1 < 0 or npre2 < 0:
+                raise RuntimeError("Error in npre1 or npre2")
+        nlst = dec[k + 1]
+        if k < nd - 1:
+            h = times[dec[k + 1] + 1] - times[dec[k + 1]]
+            nlst = find_time_layer(times, times[dec[k + 1]] - 2 * h)
+        plan.append((dec[k], dec[k + 1], npre1, npre2, nlst))
+    return dec, plan
+
+
+# ----------------------------------------------------------------------------- initial state
+def initial_state(model: Model, form: str) -> np.ndarray:
+    """(npls, n) edge DOFs of the initial field."""
+    v1 = model.edges[:, 0]
+    v2 = model.edges[:, 1]
+    axis = model.xyz[v2] - model.xyz[v1]
+    ln = np.sqrt((axis * axis).sum(1))
+    u = np.zeros((model.npls, len(v1)))
+    for ip in range(model.npls):
+        if form == "A":
+            mid = 0.5 * (model.a0[ip, v1] + model.a0[ip, v2])
+            u[ip] = (mid * axis).sum(1) / ln * ln * 0.5
+        else:
+            et = -(model.v3a[ip, v2] - model.v3a[ip, v1]) / ln
+            u[ip] = et * ln * 0.5
+    return u
+
+
+# ----------------------------------------------------------------------------- sources
+def _ref_basis(j: int, p):
+    """``Basis_func_on_reference_vec_par`` at reference point p in [-1,1]^3 (unscaled)."""
+    l0 = lambda x: (1 - x) / 2  # noqa: E731
+    l1 = lambda x: (1 + x) / 2  # noqa: E731
+    x, y, z = p
+    L = (l0, l1)
+    d = j // 4
+    k = j % 4
+    out = np.zeros(3)
+    if d == 0:
+        out[0] = L[k & 1](y) * L[k >> 1](z)
+    elif d == 1:
+        out[1] = L[k & 1](z) * L[k >> 1](x)
+    else:
+        out[2] = L[k & 1](x) * L[k >> 1](y)
+    return out
+
+
+def source_vectors(model: Model) -> np.ndarray | None:
+    """Per-source RHS shape s (npls, n): RHS += s * I(t) (A) or s * (-dI/dt) (E).
+
+    None when the current function is zero after t0 (``LoadAndInitCurrent`` returns early and
+    no dipole is placed).
+    """
+    cur = model.current
+    if cur is None or not np.any(cur[1:] != 0):
+        return None
+    n = model.edges.shape[0]
+    lo = model.xyz[model.nver].min(axis=1) - 1e-3
+    hi = model.xyz[model.nver].max(axis=1) + 1e-3
+    p0 = model.xyz[model.nver[:, 0]]
+    p7 = model.xyz[model.nver[:, 7]]
+    edlen = np.linalg.norm(model.xyz[model.edges[:, 1]] - model.xyz[model.edges[:, 0]], axis=1)
+    s = np.zeros((model.npls, n))
+    N = model.n_srs_razb
+    for ip in range(model.npls):
+        A, Bp = model.wires[ip]
+        dvec = (Bp - A) / N
+        for j in range(N):
+            p = A + (j + 0.5) * dvec
+            inside = np.all((p >= lo) & (p <= hi), axis=1)
+            cand = np.nonzero(inside)[0]
+            elem = -1
+            for e in cand:  # first element in index order that passes the local-coordinate test
+                ctr = 0.5 * (p0[e] + p7[e])
+                if ((ctr - p) ** 2).sum() <= 1e-6:   # CheckInHex keeps the start point 0.5
+                    loc = np.full(3, 0.5)
+                else:
+                    loc = (p - p0[e]) / (p7[e] - p0[e])
+                if np.all(loc > -0.01) and np.all(loc < 1.01):
+                    elem = e
+                    break
+            if elem < 0:
+                raise RuntimeError("Error in find element for sours")
+            ref = 2.0 * loc - 1.0
+            for jj in range(12):
+                ii = model.ed[elem, jj]
+                b = _ref_basis(jj, ref)
+                s[ip, ii] += (b[0] * dvec[0] + b[1] * dvec[1] + b[2] * dvec[2]) * 2.0 / edlen[ii]
+    return s
+
+
+def _dA_dt(t, u_j, u_j1, u_j2, dt, dt0, dt1, t_j, t_j1, t_j2):
+    """``Time_approx_for_vfem::dA_dt`` (``E2/time_approx.cpp:1442-1454``), literally."""
+    if t < t_j2:
+        t = t_j2
+    if t > t_j:
+        t = t_j
+    return (u_j2 * (2.0 * t - t_j - t_j1) / (dt1 * dt)
+            - u_j1 * (2.0 * t - t_j - t_j2) / (dt1 * dt0)
+            + u_j * (2.0 * t - t_j1 - t_j2) / (dt * dt0))
+
+
+def source_scale(model: Model, form: str, itime: int, dt_two_layer: float | None) -> float:
+    """Multiplier of the source shape at step ``itime`` (0 when nothing is injected)."""
+    cur = model.current
+    t = model.times
+    if form == "A":
+        return float(cur[itime]) if cur[itime] != 0 else 0.0
+    ntime = len(t)
+    dt_back = dt_two_layer if dt_two_layer is not None else (t[itime] - t[itime - 1])
+    if itime + 1 < ntime:
+        if cur[itime] == 0:
+            djdt = (cur[itime] - cur[itime - 1]) / dt_back
+        else:
+            djdt = _dA_dt(t[itime], cur[itime - 1], cur[itime], cur[itime + 1],
+                          t[itime + 1] - t[itime - 1], t[itime + 1] - t[itime], t[itime] - t[itime - 1],
+                          t[itime], t[itime - 1], t[itime - 2])
+    else:
+        djdt = (cur[itime] - cur[itime - 1]) / dt_back
+    return -djdt
+
+
+# ----------------------------------------------------------------------------- receivers
+def receiver_elements(model: Model, pts: np.ndarray) -> np.ndarray:
+    """First element (index order) whose box +-1e-6 holds each point (``FindPointsForElems``)."""
+    lo = model.xyz[model.nver[:, 0]] - 1e-6
+    hi = model.xyz[model.nver[:, 7]] + 1e-6
+    out = np.empty(len(pts), dtype=np.int64)
+    for i, p in enumerate(pts):
+        m = np.nonzero(np.all((p >= lo) & (p <= hi), axis=1))[0]
+        if len(m) == 0:
+            raise RuntimeError(f"There are no elements for receiver(s) N{i}")
+        out[i] = m[0]
+    return out
+
+
+def trilinear_coeffs(model: Model, elem: int, p) -> np.ndarray:
+    """``ScalarFieldOnParCff``: the 8 trilinear nodal weights at p in brick ``elem``."""
+    x0 = model.xyz[model.nver[elem, 0]]
+    x7 = model.xyz[model.nver[elem, 7]]
+    h = x7 - x0
+    loc = (2.0 * np.asarray(p) - x0 - x7) / h
+    l = [(1 - loc) / 2, (1 + loc) / 2]
+    c = np.empty(8)
+    for i in range(8):
+        c[i] = l[i & 1][0] * l[(i >> 1) & 1][1] * l[(i >> 2) & 1][2]
+    return c
+
+
+def subdomain(model: Model, seeds, material: int, levels: int):
+    """``Subdomain::Init``: seed elements grown ``levels`` times through shared nodes."""
+    kpar = model.kpar
+    in_el = np.zeros(kpar, bool)
+    in_nd = np.zeros(model.kuzlov, bool)
+    ok = np.ones(kpar, bool) if material < 0 else (model.nvkat == material)
+    for e in seeds:
+        if ok[e]:
+            in_el[e] = True
+            in_nd[model.nver[e]] = True
+    for _ in range(levels):
+        touch = in_nd[model.nver].any(axis=1) & ~in_el & ok
+        in_el |= touch
+        in_nd[model.nver[touch]] = True
+    return np.nonzero(in_el)[0], np.nonzero(in_nd)[0]
+
+
+def centre_value_matrix(model: Model, elems: np.ndarray, kind: str, comp: int) -> sp.csr_matrix:
+    """Rows: element-centre value (kind 'v') or curl (kind 'c') component ``comp`` as a linear
+    functional of the edge DOFs (``T_Brick::Calc_value_inside_hex``/``Calc_rotor_inside_hex``
+    at the centre)."""
+    h = fem.element_sizes(model)[elems]
+    rows, cols, vals = [], [], []
+    ed = model.ed[elems]
+    for r in range(len(elems)):
+        hx, hy, hz = h[r]
+        if kind == "v":
+            d = comp
+            hd = (hx, hy, hz)[d]
+            w = {4 * d + k: 1.0 / (2.0 * hd) for k in range(4)}
+        else:
+            if comp == 0:    # dEz/dy - dEy/dz
+                c = 1.0 / (hy * hz)
+                w = {8: -c, 9: -c, 10: c, 11: c, 4: c, 6: c, 5: -c, 7: -c}
+            elif comp == 1:  # dEx/dz - dEz/dx
+                c = 1.0 / (hx * hz)
+                w = {2: c, 3: c, 0: -c, 1: -c, 8: c, 10: c, 9: -c, 11: -c}
+            else:            # dEy/dx - dEx/dy
+                c = 1.0 / (hx * hy)
+                w = {6: c, 7: c, 4: -c, 5: -c, 0: c, 2: c, 1: -c, 3: -c}
+        for j, v in w.items():
+            rows.append(r)
+            cols.append(ed[r, j])
+            vals.append(v)
+    n = model.edges.shape[0]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(elems), n))
+
+
+def nodal_mass(model: Model, elems: np.ndarray, nodes: np.ndarray):
+    """Trilinear mass over the subdomain and the rowsums V/8 (``Hex_Local_Matrix``)."""
+    pos = -np.ones(model.kuzlov, dtype=np.int64)
+    pos[nodes] = np.arange(len(nodes))
+    h = fem.element_sizes(model)[elems]
+    vol = h.prod(axis=1)
+    loc = np.empty((8, 8))
+    for i in range(8):
+        for j in range(8):
+            nd = bin(i ^ j).count("1")
+            loc[i, j] = (1 / 27, 1 / 54, 1 / 108, 1 / 216)[nd]
+    nv = pos[model.nver[elems]]
+    rows = np.repeat(nv, 8, axis=1).reshape(-1)
+    cols = np.tile(nv, (1, 8)).reshape(-1)
+    vals = (vol[:, None, None] * loc[None]).reshape(-1)
+    M = sp.coo_matrix((vals, (rows, cols)), shape=(len(nodes), len(nodes))).tocsc()
+    return M, nv, vol / 8.0
+
+
+class ReceiverSet:
+    """Receivers of one kind: 'B' (curl of U, one subdomain) or 'E' (U, one per material)."""
+
+    def __init__(self, model: Model, pts: np.ndarray, kind: str):
+        self.model = model
+        self.pts = pts
+        self.kind = kind
+        self.elem = receiver_elements(model, pts) if len(pts) else np.zeros(0, np.int64)
+        self.cfa = np.array([trilinear_coeffs(model, e, p) for e, p in zip(self.elem, pts)])
+        self.subs = []
+        if not len(pts):
+            return
+        if kind == "B":
+            groups = [(-1, np.arange(len(pts)))]
+        else:
+            mats = sorted(set(model.nvkat[self.elem].tolist()))
+            groups = [(m, np.nonzero(model.nvkat[self.elem] == m)[0]) for m in mats]
+        for mat, ridx in groups:
+            seeds = np.unique(self.elem[ridx])
+            els, nds = subdomain(model, seeds, mat, model.n_neib)
+            M, nv, rs = nodal_mass(model, els, nds)
+            lu = spla.splu(M)
+            V = [centre_value_matrix(model, els, "c" if kind == "B" else "v", c) for c in range(3)]
+            epos = -np.ones(model.kpar, dtype=np.int64)
+            epos[els] = np.arange(len(els))
+            self.subs.append(dict(ridx=ridx, els=els, nv=nv, rs=rs, lu=lu, V=V, epos=epos,
+                                  nnodes=len(nds)))
+
+    def values(self, u: np.ndarray) -> np.ndarray:
+        """(nrec, 3) receiver values of the field with edge DOFs u (one source)."""
+        out = np.zeros((len(self.pts), 3))
+        for s in self.subs:
+            for c in range(3):
+                cen = s["V"][c] @ u
+                rhs = np.zeros(s["nnodes"])
+                np.add.at(rhs, s["nv"].reshape(-1), np.repeat(s["rs"] * cen, 8))
+                q = s["lu"].solve(rhs)
+                for r in s["ridx"]:
+                    e = s["epos"][self.elem[r]]
+                    out[r, c] = q[s["nv"][e]] @ self.cfa[r]
+        return out
+
+
+# ----------------------------------------------------------------------------- run
+class Result:
+    pass
+
+
+def print_window(model: Model):
+    """``time_layer_first``/``time_layer_last`` (``A2/give_out_vec_loop.cpp:46-73``)."""
+    t = model.times
+    nt = len(t)
+    left, right = model.print_window
+    first = last = 2
+    for i in range(nt):
+        if t[i] >= left:
+            first = i
+            break
+    if first > nt - 1:
+        first = nt - 1
+    for i in range(nt):
+        if t[i] >= right - 1e-6:
+            last = i
+            break
+    if right > t[nt - 1] - 1e-6:
+        last = nt - 1
+    return first, last
+
+
+def run(model: Model, form: str, scheme: int, solver=None, keep_fields=False) -> Result:
+    """Run the whole calculation; returns receiver values per layer and the derived tables.
+
+    ``solver(K_csc) -> callable(b)`` may replace SuperLU (used to inject other solvers).
+    """
+    t = model.times
+    nt = len(t)
+    n = model.edges.shape[0]
+    ig, jg = fem.portrait(model.ed, n)
+    b_ggl, b_di, c_ggl, c_di = fem.assemble(model, ig, jg)
+    bnd = fem.boundary_edges(model)
+    u0 = initial_state(model, form)
+    src = source_vectors(model) if model.fdirect else None
+    first, last = print_window(model)
+    recB = ReceiverSet(model, model.recB, "B")
+    recE = ReceiverSet(model, model.recE, "E")
+    nout = last - first + 1
+    valB = np.zeros((nout, model.npls, len(model.recB), 3))
+    valE = np.zeros((nout, model.npls, len(model.recE), 3))
+    output_mask = np.zeros(nt, bool)
+    dec, plan = decade_plan(t)
+    vpre1 = np.full(nt, -1)
+    vpre2 = np.full(nt, -2)
+    factor = solver or (lambda K: spla.splu(K).solve)
+
+    def factorize(mult):
+        ggl, di = fem.system_matrix(ig, jg, b_ggl, b_di, c_ggl, c_di, mult, bnd)
+        return factor(fem.rsf_to_csc(ig, jg, ggl, di))
+
+    Cfull = fem.rsf_to_csc(ig, jg, c_ggl, c_di).tocsr()
+
+    def rhs(u1, u2, m1, m2, scale):
+        x = u1 * m1 + (u2 * m2 if m2 != 0.0 else 0.0)
+        pr = (Cfull @ x.T).T
+        if scale != 0.0 and src is not None:
+            pr = pr + src * scale
+        pr[:, bnd] = 0.0
+        return pr
+
+    def output(layer, u):
+        if first <= layer <= last:
+            output_mask[layer] = True
+            for ip in range(model.npls):
+                valB[layer - first, ip] = recB.values(u[ip]) if len(model.recB) else 0
+                valE[layer - first, ip] = recE.values(u[ip]) if len(model.recE) else 0
+
+    stored = {}
+    fields = {} if keep_fields else None
+    for k, (tbeg, tend, npre1, npre2, nlst) in enumerate(plan):
+        solve = None
+        if 1 >= tbeg and 1 < tend:
+            u_j = u0.copy()
+            u_j1 = u0.copy()
+            u_j2 = u0.copy()
+            t_j = t_j1 = t[0]
+            tn = tn1 = 0
+            if scheme == 3:
+                # Two_Layers_Schema: steps 1 and 2 with dt = t1 - t0, no receiver output
+                dtl = t[1] - t[0]
+                solve2 = factorize(1.0 / dtl)
+                for it in (1, 2):
+                    sc = source_scale(model, form, it, dtl) if src is not None else 0.0
+                    pr = rhs(u_j, u_j, 1.0 / dtl, 0.0, sc)
+                    u_j = np.stack([solve2(pr[ip]) for ip in range(model.npls)])
+                    if it >= nlst:
+                        stored[it] = u_j.copy()
+                    vpre1[it] = it - 1
+                    vpre2[it] = it - 2
+                    if keep_fields:
+                        fields[it] = u_j.copy()
+                    u_j2 = u_j1
+                    u_j1 = u_j.copy()
+                t_j2, t_j1, t_j = t[0], t[1], t[2]
+                tn2, tn1, tn = 0, 1, 2
+            start = 1 if scheme == 2 else 3
+        else:
+            u_j = np.zeros_like(u0)
+            u_j1 = np.zeros_like(u0)
+            u_j2 = np.zeros_like(u0)
+            start = tbeg
+        fcalc = False
+        first_step = True
+        for it in range(max(start, tbeg), tend):
+            fsdiff = False
+            if not fcalc and it > (1 if scheme == 2 else 3):
+                u_j1 = stored[npre1].copy()
+                u_j2 = stored[npre2].copy()
+                fsdiff = True
+            if not fsdiff:
+                t_j2, t_j1, t_j = t_j1, t_j, t[it]
+                tn2, tn1, tn = tn1, tn, it
+            else:
+                t_j2, t_j1, t_j = t[npre2], t[npre1], t[it]
+                tn2, tn1, tn = npre2, npre1, it
+            vpre1[it] = tn1
+            vpre2[it] = tn2
+            if first_step:
+                dt = t_j - t_j2
+                dt0 = t_j - t_j1
+                dt1 = t_j1 - t_j2
+                if scheme == 2:
+                    mult = 1.0 / (t_j - t_j1)
+                    mult1, mult2 = mult, 0.0
+                first_step = False
+            if scheme == 3:
+                mult = (dt + dt0) / (dt * dt0)
+                mult1 = dt / (dt1 * dt0)
+                mult2 = -dt0 / (dt * dt1)
+            if not fcalc:
+                solve = factorize(mult)
+            sc = source_scale(model, form, it, None) if src is not None else 0.0
+            pr = rhs(u_j1, u_j2, mult1, mult2, sc)
+            u_j = np.stack([solve(pr[ip]) for ip in range(model.npls)])
+            if it >= nlst:
+                stored[it] = u_j.copy()
+            if keep_fields:
+                fields[it] = u_j.copy()
+            if tn1 < last:
+                output(tn1, u_j1)
+            if tn == last:
+                output(tn, u_j)
+            u_j2 = u_j1
+            u_j1 = u_j
+            fcalc = True
+    res = Result()
+    res.first, res.last = first, last
+    res.valB, res.valE = valB, valE
+    res.output_mask = output_mask
+    res.vpre1, res.vpre2 = vpre1, vpre2
+    res.dec, res.plan = dec, plan
+    res.fields = fields
+    res.eds, res.bfield, res.efield = gather(model, form, scheme, first, last, valB, valE, vpre1, vpre2)
+    return res
+
+
+def _diff_series(t, F, first, last, vpre1, vpre2, scheme):
+    """Time derivative of F (ntout, ...) per ``Gather_edsall`` (2P/3P variants)."""
+    out = np.zeros_like(F)
+    idx = lambda j: j - first  # noqa: E731
+    j, j1, j2 = first, first + 1, first + 2
+    dt = t[j2] - t[j]
+    dt1 = t[j1] - t[j]
+    dt0 = t[j2] - t[j1]
+    out[idx(j)] = F[idx(j)] * (-(dt + dt1) / (dt1 * dt)) + F[idx(j1)] * (dt / (dt1 * dt0)) \
+        + F[idx(j2)] * (-dt1 / (dt * dt0))
+    while True:
+        if scheme == 2:
+            m0 = -1.0 / (t[j1] - t[j])
+            m1 = 1.0 / (t[j1] - t[j])
+            out[idx(j1)] = F[idx(j)] * m0 + F[idx(j1)] * m1 + F[idx(j2)] * 0.0
+        else:
+            m0 = -dt0 / (dt1 * dt)
+            m1 = -(dt1 - dt0) / (dt1 * dt0)
+            m2 = dt1 / (dt * dt0)
+            out[idx(j1)] = F[idx(j)] * m0 + F[idx(j1)] * m1 + F[idx(j2)] * m2
+        if j2 == last:
+            break
+        if scheme == 2:
+            j2 += 1
+            j = vpre2[j2]
+            j1 = vpre1[j2]
+        else:
+            j = j1
+            j2 += 1
+            j1 = vpre1[j2]
+        dt = t[j2] - t[j]
+        dt1 = t[j1] - t[j]
+        dt0 = t[j2] - t[j1]
+    if scheme == 2:
+        m1 = -1.0 / (t[j2] - t[j1])
+        m2 = 1.0 / (t[j2] - t[j1])
+        out[idx(j2)] = F[idx(j)] * 0.0 + F[idx(j1)] * m1 + F[idx(j2)] * m2
+    else:
+        out[idx(j2)] = F[idx(j)] * (dt0 / (dt1 * dt)) + F[idx(j1)] * (-dt / (dt1 * dt0)) \
+            + F[idx(j2)] * ((dt + dt0) / (dt * dt0))
+    return out
+
+
+def gather(model, form, scheme, first, last, valB, valE, vpre1, vpre2):
+    """``Gather_edsall``: returns (Eds, B, E) arrays as the output files hold them (before
+    the x1000 / x(-1000) output scaling)."""
+    t = model.times
+    if form == "A":
+        eds = _diff_series(t, valB, first, last, vpre1, vpre2, scheme) if valB.shape[2] else valB
+        efield = -_diff_series(t, valE, first, last, vpre1, vpre2, scheme) if valE.shape[2] else valE
+        return eds, valB, efield
+    eds = -valB
+    return eds, None, valE

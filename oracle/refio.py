@@ -1,0 +1,195 @@
+"""Readers of the reference's input files and writers of its output tables (oracle side).
+
+Follows ``A2/vec_prep_data.cpp:83-257`` (materials, receivers, mesh), ``A2/in_out.cpp:222-294``
+(``inftry.dat``), ``A2/vec_prep_data.cpp:834-853`` (``infite.0``), ``A2/time_approx.cpp:243-287``
+(edges), ``A2/TaskSP.cpp:467-535`` (group / receiver counts),
+``A2/give_out_vec_loop.cpp:46-73`` (print window) and ``A2/vec_prep_data.cpp:982-1045``
+(current function, line source).
+"""
+from __future__ import annotations
+
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+MU_0 = 1.25663706143591729e-6  # A2/vfem_const.h:21
+
+
+@dataclass
+class Model:
+    kuzlov: int
+    kpar: int
+    xyz: np.ndarray        # (kuzlov, 3)
+    nver: np.ndarray       # (kpar, 8) 0-based
+    nvkat: np.ndarray      # (kpar,) 0-based material
+    sigma: np.ndarray      # per material
+    sigma0: np.ndarray
+    mu: np.ndarray         # absolute permeability per material
+    l13d: np.ndarray       # 0-based boundary nodes
+    edges: np.ndarray      # (n, 2) 0-based nodes
+    ed: np.ndarray         # (kpar, 12) 0-based edges
+    times: np.ndarray
+    recB: np.ndarray
+    recE: np.ndarray
+    npls: int
+    rec_per_src_B: np.ndarray
+    rec_per_src_E: np.ndarray
+    fdirect: int
+    current: np.ndarray | None
+    wires: list
+    n_srs_razb: int
+    n_neib: int
+    print_window: tuple
+    a0: np.ndarray | None   # (npls, kuzlov, 3)
+    v3a: np.ndarray | None  # (npls, kuzlov)
+
+
+def _read_flag(path, default, cast=float):
+    try:
+        with open(path) as f:
+            tok = f.read().split()
+        return cast(tok[0]) if tok else default
+    except OSError:
+        return default
+
+
+def read_material_table(path):
+    """Rows ``idx a b``; returns arrays indexed by idx-1 (``vec_prep_data.cpp:96-165``)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) >= 3:
+                rows.append((int(p[0]), float(p[1]), float(p[2])))
+    nmat = max(r[0] for r in rows)
+    a = np.zeros(nmat)
+    b = np.zeros(nmat)
+    for i, x, y in rows:
+        a[i - 1] = x
+        b[i - 1] = y
+    return a, b
+
+
+def read_inftry(path):
+    with open(path) as f:
+        txt = f.read()
+    vals = {}
+    for key in ("KUZLOV=", "KPAR=", "KT1="):
+        m = re.search(re.escape(key) + r"\s*(-?\d+)", txt)
+        vals[key] = int(m.group(1))
+    return vals["KUZLOV="], vals["KPAR="], vals["KT1="]
+
+
+def read_infite0(path):
+    """``Read_infite0``: skip to '=', read ntime, skip 4 lines, read ';'-separated times."""
+    with open(path) as f:
+        txt = f.read()
+    i = txt.index("=")
+    m = re.match(r"\s*(\d+)", txt[i + 1:])
+    ntime = int(m.group(1))
+    rest = txt[i + 1:]
+    for _ in range(4):
+        rest = rest[rest.index("\n") + 1:]
+    vals = [v for v in rest.replace("\n", " ").split(";") if v.strip()]
+    return np.array([float(v) for v in vals[:ntime]])
+
+
+def read_points(path):
+    with open(path) as f:
+        tok = f.read().split()
+    n = int(tok[0])
+    return np.array([float(t) for t in tok[1:1 + 3 * n]]).reshape(n, 3)
+
+
+def read_model(d: str) -> Model:
+    j = lambda name: os.path.join(d, name)  # noqa: E731
+    kuz, kpar, kt1 = read_inftry(j("inftry.dat"))
+    xyz = np.fromfile(j("xyz.dat"), dtype="<f8", count=3 * kuz).reshape(kuz, 3)
+    nver = np.fromfile(j("nver.dat"), dtype="<i4", count=14 * kpar).reshape(kpar, 14)[:, :8] - 1
+    nvkat = np.fromfile(j("nvkat.dat"), dtype="<i4", count=kpar) - 1
+    lname = j("l13d.dat") if os.path.exists(j("l13d.dat")) else j("L13d.dat")
+    l13d = np.fromfile(lname, dtype="<i4", count=kt1) - 1
+    with open(j("tsize3d_.dat")) as f:
+        n = int(f.read().split()[1])
+    edges = np.fromfile(j("nodesforedges.dat"), dtype="<i4", count=2 * n).reshape(n, 2) - 1
+    ed = np.fromfile(j("edges.dat"), dtype="<i4", count=25 * kpar).reshape(kpar, 25)[:, :12] - 1
+    sig, sig0 = read_material_table(j("sig3d"))
+    mur, _mur0 = read_material_table(j("mu3d"))
+    mu = mur * MU_0
+    times = read_infite0(j("infite.0"))
+    recB = read_points(j("xyzVectorB"))
+    recE = read_points(j("xyzVectorE"))
+    with open(j("group")) as f:
+        tok = f.read().split()
+    ngrp = int(tok[0])
+    npls = sum(int(tok[3 + 3 * g]) - int(tok[2 + 3 * g]) + 1 for g in range(ngrp))
+    with open(j("recvsb")) as f:
+        rb = np.array([int(t) for t in f.read().split()[:npls]])
+    with open(j("recvse")) as f:
+        re_ = np.array([int(t) for t in f.read().split()[:npls]])
+    fdirect = int(_read_flag(j("fdirect"), 0, int))
+    current = None
+    if os.path.exists(j("currentfunction")):
+        cf = np.loadtxt(j("currentfunction")).reshape(-1, 2)
+        current = cf[:len(times), 1].copy()
+    wires = []
+    if os.path.exists(j("sours")):
+        with open(j("sours")) as f:
+            tok = [float(t) for t in f.read().split()]
+        for i in range(npls):
+            wires.append((np.array(tok[6 * i:6 * i + 3]), np.array(tok[6 * i + 3:6 * i + 6])))
+    with open(j("timeintervalforprint")) as f:
+        tok = f.read().split()
+    window = (float(tok[0]), float(tok[1]))
+    a0 = v3a = None
+    if os.path.exists(j("a0.dat")):
+        a0 = np.fromfile(j("a0.dat"), dtype="<f8", count=3 * kuz * npls).reshape(npls, kuz, 3)
+    if os.path.exists(j("v3a.dat")):
+        v3a = np.fromfile(j("v3a.dat"), dtype="<f8", count=kuz * npls).reshape(npls, kuz)
+    return Model(kuz, kpar, xyz, nver, nvkat, sig, sig0, mu, l13d, edges, ed, times, recB, recE,
+                 npls, rb, re_, fdirect, current, wires,
+                 int(_read_flag(j("nSrsRazb"), 500, int)), int(_read_flag(j("n_neib"), 5, int)),
+                 window, a0, v3a)
+
+
+def write_tables(d: str, model: Model, t_first: int, t_last: int, eds, bfield, efield, form: str):
+    """``Gather_edsall`` output writers (``A2/give_out_vec_loop.cpp:532-609``).
+
+    ``eds``/``bfield``/``efield``: arrays (ntout, npls, nrec, 3) already differentiated.
+    """
+    recB, recE = model.recB, model.recE
+    igB = np.concatenate([[0], np.cumsum(model.rec_per_src_B)])
+    igE = np.concatenate([[0], np.cumsum(model.rec_per_src_E)])
+    t = model.times
+
+    def fmt(v):
+        return f"{v:.16e}"
+
+    for ipls in range(model.npls):
+        for fname, recs, ig, arr, scale, head in (
+            (f"edsall_anom.{ipls + 1}", recB, igB, eds, -1000.0,
+             "      t (ms)         Emfx_anom (mV)     Emfy_anom (mV)     Emfz_anom (mV)\n"),
+            (f"ball_anom.{ipls + 1}", recB, igB, bfield, 1000.0,
+             "      t (ms)        Bx_anom (mTl)     By_anom (mTl)     Bz_anom (mTl)\n"),
+            (f"eall_anom.{ipls + 1}", recE, igE, efield, 1000.0,
+             "      t (ms)         Ex_anom n (mV)     Ey_anom a (mV)     Ez_anom s (mV)\n"),
+        ):
+            with open(os.path.join(d, fname), "w") as f:
+                for i in range(len(recs)):
+                    if i < ig[ipls] or i >= ig[ipls + 1]:
+                        continue
+                    f.write(f"{int(recs[i][0])} {int(recs[i][1])}\n")
+                    f.write(f"{fmt(recs[i][0])} {fmt(recs[i][1])} {fmt(recs[i][2])}\n")
+                    f.write("1\n1 1 1 1\n1\n")
+                    f.write(head)
+                    for jt in range(t_first, t_last + 1):
+                        k = jt - t_first
+                        if arr is None:
+                            vals = (0.0, 0.0, 0.0)
+                        else:
+                            vals = tuple(arr[k, ipls, i, c] * scale for c in range(3))
+                        f.write(f"{fmt(t[jt] * 1000)}\t{fmt(vals[0])}\t{fmt(vals[1])}\t{fmt(vals[2])}\n")
+                if len(recs) == 0:
+                    f.write(" \n")

@@ -1,0 +1,221 @@
+"""Pins the oracle's time stepping, sources and receivers (CPU only)."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from conftest import make_model
+from oracle import calc, fem, refio
+
+
+def test_decade_plan_known_answer():
+    t = np.array([0, 1, 2, 3, 4, 6, 8, 10, 14, 18, 22], float)
+    dec, plan = calc.decade_plan(t)
+    assert dec == [0, 5, 8, 11]
+    assert plan[0] == (0, 5, -1, -1, 2)
+    assert plan[1] == (5, 8, 4, 2, 5)
+    assert plan[2] == (8, 11, 7, 5, 11)
+
+
+def test_find_time_layer_tolerance():
+    t = np.array([0.0, 1e-3, 2e-3, 3.0000001e-3])
+    assert calc.find_time_layer(t, 3e-3) == 3        # 3.3e-8 relative < 1e-6
+    assert calc.find_time_layer(t, 3.01e-3) == -1
+    assert calc.find_time_layer(t, 0.0) == 0
+
+
+def _eigen_case(tmp_path, scheme):
+    """Small model whose initial field is a generalized eigenvector of (B, C) with Dirichlet."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(4, 4, 4), steps=(6, 2), scheme=scheme)
+    m = refio.read_model(d)
+    n = m.edges.shape[0]
+    ig, jg = fem.portrait(m.ed, n)
+    b_ggl, b_di, c_ggl, c_di = fem.assemble(m, ig, jg)
+    bnd = fem.boundary_edges(m)
+    free = np.nonzero(~bnd)[0]
+    B = fem.rsf_to_csc(ig, jg, b_ggl, b_di)[free][:, free].toarray()
+    C = fem.rsf_to_csc(ig, jg, c_ggl, c_di)[free][:, free].toarray()
+    import scipy.linalg as sl
+    w, V = sl.eigh(B, C)
+    k = int(np.searchsorted(w, 1e3 * w.max() / 1e6))  # a curl mode, well above the gradient cluster
+    lam = w[k]
+    u0 = np.zeros(n)
+    u0[free] = V[:, k]
+    return m, lam, u0
+
+
+@pytest.mark.parametrize("scheme", [2, 3])
+def test_scheme_reproduces_exact_discrete_ode(tmp_path, scheme):
+    """u0 = eigenvector v: every step must give v * a_n with the textbook scalar recurrence
+    (backward Euler, then BDF2 on a uniform grid) - pins mult/mult1/mult2 and the restarts."""
+    m, lam, u0 = _eigen_case(tmp_path, scheme)
+    orig = calc.initial_state
+    calc.initial_state = lambda model, form: u0[None].copy()
+    try:
+        r = calc.run(m, "A", scheme, keep_fields=True)
+    finally:
+        calc.initial_state = orig
+    t = m.times
+    # the 1e-12 diagonal bump perturbs the eigenvalue problem at the 1e-12 level only
+    a = {0: 1.0}
+    dec, plan = calc.decade_plan(t)
+    for (tbeg, tend, npre1, npre2, nlst) in plan:
+        for it in range(max(tbeg, 1), tend):
+            if scheme == 2 or it <= 2:
+                prev = npre1 if (it == tbeg and tbeg > 1) else it - 1
+                h = t[it] - t[prev] if scheme == 2 else t[1] - t[0]
+                a[it] = a[prev] / (1.0 + lam * h)
+            else:
+                if it == tbeg and tbeg > 3:
+                    p1, p2 = npre1, npre2
+                else:
+                    p1, p2 = it - 1, it - 2
+                h = t[it] - t[p1]
+                a[it] = (2.0 * a[p1] - 0.5 * a[p2]) / (1.5 + lam * h)
+    for it, u in r.fields.items():
+        want = a[it] * u0
+        err = np.abs(u[0] - want).max() / np.abs(u0).max()
+        assert err < 1e-9, (it, err)
+
+
+def test_static_gradient_field_is_preserved(tmp_path):
+    """A discrete gradient vanishing on the boundary is in ker(B): C u' = 0 keeps it constant."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(4, 4, 4), steps=(4, 2), scheme=3)
+    m = refio.read_model(d)
+    m.sigma[:] = 1.0   # keeps the 1e-12 diagonal bump negligible against mult*C in every cell
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal(m.kuzlov)
+    phi[m.l13d] = 0.0
+    u0 = 0.5 * (phi[m.edges[:, 1]] - phi[m.edges[:, 0]])
+    orig = calc.initial_state
+    calc.initial_state = lambda model, form: u0[None].copy()
+    try:
+        r = calc.run(m, "A", 3, keep_fields=True)
+    finally:
+        calc.initial_state = orig
+    for it, u in r.fields.items():
+        assert np.abs(u[0] - u0).max() < 1e-8 * np.abs(u0).max(), it
+
+
+def _uniform_field_dofs(m, E0):
+    t = m.xyz[m.edges[:, 1]] - m.xyz[m.edges[:, 0]]
+    return (t @ np.asarray(E0)) * 0.5
+
+
+def _constant_curl_dofs(m, B0):
+    mid = 0.5 * (m.xyz[m.edges[:, 1]] + m.xyz[m.edges[:, 0]])
+    A = 0.5 * np.cross(np.asarray(B0)[None], mid)
+    t = m.xyz[m.edges[:, 1]] - m.xyz[m.edges[:, 0]]
+    return (A * t).sum(1) * 0.5
+
+
+def test_receivers_reproduce_uniform_fields(tmp_path):
+    d, _ = make_model(tmp_path, cfg=1, cells=(8, 8, 6))
+    m = refio.read_model(d)
+    E0 = np.array([0.3, -1.2, 0.7])
+    B0 = np.array([-0.4, 0.9, 2.5])
+    rE = calc.ReceiverSet(m, m.recE, "E")
+    rB = calc.ReceiverSet(m, m.recB, "B")
+    vE = rE.values(_uniform_field_dofs(m, E0))
+    np.testing.assert_allclose(vE, np.broadcast_to(E0, vE.shape), rtol=1e-10, atol=1e-12)
+    assert np.abs(rB.values(_uniform_field_dofs(m, E0))).max() < 1e-10
+    vB = rB.values(_constant_curl_dofs(m, B0))
+    np.testing.assert_allclose(vB, np.broadcast_to(B0, vB.shape), rtol=1e-10, atol=1e-12)
+
+
+def test_centre_curl_matches_finite_difference(tmp_path):
+    d, _ = make_model(tmp_path, cfg=1, cells=(3, 3, 3))
+    m = refio.read_model(d)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(m.edges.shape[0])
+    e = 13
+    h = fem.element_sizes(m)[e]
+    x0 = m.xyz[m.nver[e, 0]]
+
+    def field(p):
+        s = (p - x0) / h
+        out = np.zeros(3)
+        for a in range(12):
+            dd = a // 4
+            k = a % 4
+            pp, qq = (dd + 1) % 3, (dd + 2) % 3
+            l = lambda b, x: x if b else 1 - x  # noqa: E731
+            out[dd] += 2 / h[dd] * l(k & 1, s[pp]) * l(k >> 1, s[qq]) * u[m.ed[e, a]]
+        return out
+
+    c = x0 + 0.5 * h
+    eps = 1e-4 * h
+    J = np.zeros((3, 3))
+    for j in range(3):
+        dp = np.zeros(3)
+        dp[j] = eps[j]
+        J[:, j] = (field(c + dp) - field(c - dp)) / (2 * eps[j])
+    curl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+    for comp in range(3):
+        row = calc.centre_value_matrix(m, np.array([e]), "c", comp)
+        assert np.isclose((row @ u)[0], curl[comp], rtol=1e-8, atol=1e-10)
+        rowv = calc.centre_value_matrix(m, np.array([e]), "v", comp)
+        assert np.isclose((rowv @ u)[0], field(c)[comp], rtol=1e-12)
+
+
+def _restart_grid():
+    t = [0.0]
+    for h, k in ((1.0, 5), (2.0, 4), (4.0, 4)):
+        for _ in range(k):
+            t.append(t[-1] + h)
+    return np.array(t)
+
+
+@pytest.mark.parametrize("scheme,deg", [(2, 1), (3, 2)])
+def test_time_differences_exact(scheme, deg):
+    """2-point differences are exact on linear signals, 3-point ones on quadratics, across
+    decade restarts (vPre tables from the decade plan)."""
+    t = _restart_grid()
+    dec, plan = calc.decade_plan(t)
+    nt = len(t)
+    vpre1 = np.full(nt, -1)
+    vpre2 = np.full(nt, -2)
+    for (tbeg, tend, n1, n2, _) in plan:
+        for it in range(max(tbeg, 1), tend):
+            if it == tbeg and tbeg > 1:
+                vpre1[it], vpre2[it] = n1, n2
+            else:
+                vpre1[it], vpre2[it] = it - 1, it - 2
+    coef = [0.3, -1.1, 0.25][:deg + 1]
+    F = sum(c * t ** k for k, c in enumerate(coef))
+    dF = sum(k * c * t ** (k - 1) for k, c in enumerate(coef) if k)
+    first, last = 1, nt - 1
+    out = calc._diff_series(t, F[first:last + 1, None], first, last, vpre1, vpre2, scheme)
+    np.testing.assert_allclose(out[:, 0], dF[first:last + 1], rtol=1e-12, atol=1e-12)
+
+
+def test_centre_dipole_source_weights(tmp_path):
+    """A dipole at an element centre puts 1/4 of dl on each of the 4 parallel edges."""
+    cur = lambda nt: np.where(np.arange(nt) < 2, 1.0, 0.0)  # noqa: E731
+    d, spec = make_model(tmp_path, cfg=1, cells=(4, 4, 4), steps=(3, 2), current=cur)
+    m = refio.read_model(d)
+    e = 21
+    x0, x7 = m.xyz[m.nver[e, 0]], m.xyz[m.nver[e, 7]]
+    c = 0.5 * (x0 + x7) + np.array([0.37, 0.0, 0.0])   # off the exact centre (CheckInHex start)
+    L = 1e-3
+    m.wires = [(c - [L / 2, 0, 0], c + [L / 2, 0, 0])]
+    m.n_srs_razb = 1
+    s = calc.source_vectors(m)[0]
+    edlen = np.linalg.norm(m.xyz[m.edges[:, 1]] - m.xyz[m.edges[:, 0]], axis=1)
+    nz = np.nonzero(s)[0]
+    assert sorted(nz.tolist()) == sorted(m.ed[e, :4].tolist())
+    np.testing.assert_allclose(s[nz] * edlen[nz] / 2, L / 4, rtol=1e-12)
+
+
+def test_superlu_matches_dense(tmp_path):
+    """The oracle's sparse solve (SuperLU) against dense LAPACK on a small decade matrix."""
+    d, _ = make_model(tmp_path, cfg=1, cells=(5, 5, 4))
+    m = refio.read_model(d)
+    n = m.edges.shape[0]
+    ig, jg = fem.portrait(m.ed, n)
+    b_ggl, b_di, c_ggl, c_di = fem.assemble(m, ig, jg)
+    ggl, di = fem.system_matrix(ig, jg, b_ggl, b_di, c_ggl, c_di, 1e5, fem.boundary_edges(m))
+    K = fem.rsf_to_csc(ig, jg, ggl, di)
+    b = np.random.default_rng(0).standard_normal(n)
+    x = spla.splu(K).solve(b)
+    xd = np.linalg.solve(K.toarray(), b)
+    assert np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
