@@ -53,14 +53,17 @@ class CalcInfo(C.Structure):
                 ("n_rec_b", C.c_int64), ("n_rec_e", C.c_int64), ("setup_seconds", C.c_double),
                 ("factor_seconds", C.c_double), ("step_seconds", C.c_double), ("steps", C.c_int64),
                 ("nnz_l", C.c_double), ("factor_flops", C.c_double), ("refine_iterations", C.c_int64),
-                ("fallbacks", C.c_int64)]
+                ("fallbacks", C.c_int64), ("factorizations", C.c_int64), ("factor_gpu_seconds", C.c_double),
+                ("solves", C.c_int64), ("timed_solves", C.c_int64), ("solve_gpu_seconds", C.c_double),
+                ("n_factored", C.c_int64), ("n_fronts", C.c_int64), ("n_levels", C.c_int64),
+                ("c_nnz", C.c_int64), ("nnz_low", C.c_int64), ("done", C.c_int)]
 
 
 EXPORTS = [
     "emtd_last_error", "emtd_portrait", "emtd_assemble",
     "emtd_solver_create", "emtd_solver_factorize", "emtd_solver_solve_nrhs", "emtd_solver_stop",
     "emtd_solver_destroy", "emtd_solver_get_stats",
-    "emtd_calc_create", "emtd_calc_run", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
+    "emtd_calc_create", "emtd_calc_run", "emtd_calc_step", "emtd_calc_set_timing", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
     "emtd_calc_last_field", "emtd_calc_get_info", "emtd_calc_destroy", "emtd_calc_nonstat_3d_line",
 ]
 
@@ -175,6 +178,15 @@ class Calc:
 
     def run(self):
         _check(lib().emtd_calc_run(self.h))
+
+    def step(self, nsteps: int = 1) -> int:
+        """Advance by up to nsteps time steps; returns how many were taken."""
+        k = C.c_int(0)
+        _check(lib().emtd_calc_step(self.h, C.c_int(nsteps), C.byref(k)))
+        return k.value
+
+    def set_timing(self, on: bool = True):
+        _check(lib().emtd_calc_set_timing(self.h, C.c_int(1 if on else 0)))
 
     def write_outputs(self):
         _check(lib().emtd_calc_write_outputs(self.h))

@@ -182,6 +182,7 @@ CalcCore::~CalcCore() {
         c64_.reset();
         c32_.reset();
         stored_.clear();
+        for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
         if (stream_) (void)hipStreamDestroy(stream_);
     } catch (...) {
     }
@@ -201,15 +202,46 @@ void CalcCore::factorize(double mult) {
     }
     if (bad >= 0) throw Error(EMTD_ERR_NOTSPD, "non-positive pivot at permuted index " + std::to_string(bad));
     factor_seconds_ += wall_seconds() - t0;
+    factor_gpu_seconds_ += c64_ ? c64_->last_factor_seconds() : c32_->last_factor_seconds();
     ++n_factorizations_;
 }
 
 void CalcCore::solve(const double* b, double* x) {
+    if (timing_) {
+        if (ev_used_ + 2 > ev_.size()) {
+            hipEvent_t e;
+            for (int k = 0; k < 2; ++k) { EMTD_HIP(hipEventCreate(&e)); ev_.push_back(e); }
+        }
+        EMTD_HIP(hipEventRecord(ev_[ev_used_], stream_));
+    }
     if (c64_)
         c64_->solve(b, x, npls_, n_);
     else
         c32_->solve(b, x, npls_, n_);
+    if (timing_) {
+        EMTD_HIP(hipEventRecord(ev_[ev_used_ + 1], stream_));
+        ev_used_ += 2;
+    }
     launch_zero_tail(stream_, x, nn_, n_, npls_);
+    ++n_solves_;
+}
+
+void CalcCore::set_timing(bool on) {
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    collect_solve_times();
+    timing_ = on;
+}
+
+void CalcCore::collect_solve_times() {
+    if (ev_used_ == 0) return;
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    for (size_t k = 0; k + 1 < ev_used_; k += 2) {
+        float ms = 0;
+        EMTD_HIP(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
+        solve_gpu_seconds_ += ms * 1e-3;
+        ++timed_solves_;
+    }
+    ev_used_ = 0;
 }
 
 double CalcCore::source_scale(int itime, double dt_two_layer) const {
@@ -255,128 +287,182 @@ double* CalcCore::keep_layer(int layer, const double* u) {
     return it->second.p;
 }
 
-void CalcCore::run() {
-    const std::vector<double>& t = M_.times;
-    const int nt = int(t.size());
+void CalcCore::begin() {
     const size_t vn = size_t(n_) * npls_;
-    DevBuf<double> bufA(vn), bufB(vn), bufC(vn), rhs(vn);
-    double* uj = bufA.p;
-    double* uj1 = bufB.p;
-    double* uj2 = bufC.p;
-    vpre1_.assign(nt, -1);
-    vpre2_.assign(nt, -2);
-    const double ts = wall_seconds();
-    double t_factor0 = factor_seconds_;
-    for (size_t k = 0; k < plan_.size(); ++k) {
-        const Decade& D = plan_[k];
-        double t_j = 0, t_j1 = 0, t_j2 = 0;
-        int tn = 0, tn1 = 0, tn2 = 0;
-        int start;
-        if (1 >= D.tbeg && 1 < D.tend) {
-            for (double* p : {uj, uj1, uj2})
-                EMTD_HIP(hipMemcpyAsync(p, u0_.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-            t_j = t_j1 = t[0];
-            tn = tn1 = 0;
-            if (scheme_ == 3) {
-                // Two_Layers_Schema: steps 1 and 2 with dt = t1 - t0, no receiver output
-                const double dtl = t[1] - t[0];
-                factorize(1.0 / dtl);
-                for (int it = 1; it <= 2 && it < nt; ++it) {
-                    launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, uj, uj, 1.0 / dtl, 0.0,
-                               src_.n ? src_.p : nullptr, source_scale(it, dtl), rhs.p);
-                    solve(rhs.p, uj2);  // solution into the free buffer
-                    std::swap(uj, uj2);  // uj = new solution, uj2 = previous u_j
-                    if (it >= D.nlst) keep_layer(it, uj);
-                    vpre1_[it] = it - 1;
-                    vpre2_[it] = it - 2;
-                    // shift: u_j2 <- u_j1, u_j1 <- u_j
-                    EMTD_HIP(hipMemcpyAsync(uj2, uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-                    EMTD_HIP(hipMemcpyAsync(uj1, uj, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-                    ++steps_;
-                }
-                t_j2 = t[0];
-                t_j1 = t[1];
-                t_j = t[2];
-                tn2 = 0;
-                tn1 = 1;
-                tn = 2;
-                start = 3;
-            } else {
-                start = 1;
-            }
-        } else {
-            start = D.tbeg;
-        }
-        bool fcalc = false, first_step = true;
-        double mult = 0, mult1 = 0, mult2 = 0, dt = 0, dt0 = 0, dt1 = 0;
-        for (int it = std::max(start, D.tbeg); it < D.tend; ++it) {
-            if (stop_requested()) throw Error(EMTD_ERR_STOPPED, "stop file found");
-            bool fsdiff = false;
-            if (!fcalc && it > (scheme_ == 2 ? 1 : 3)) {
-                auto a = stored_.find(D.npre1), b = stored_.find(D.npre2);
-                if (a == stored_.end() || b == stored_.end()) throw Error(EMTD_ERR_NOFILE, "restart layer missing");
-                EMTD_HIP(hipMemcpyAsync(uj1, a->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-                EMTD_HIP(hipMemcpyAsync(uj2, b->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-                fsdiff = true;
-            }
-            if (!fsdiff) {
-                t_j2 = t_j1; t_j1 = t_j; t_j = t[it];
-                tn2 = tn1; tn1 = tn; tn = it;
-            } else {
-                t_j2 = t[D.npre2]; t_j1 = t[D.npre1]; t_j = t[it];
-                tn2 = D.npre2; tn1 = D.npre1; tn = it;
-            }
-            vpre1_[it] = tn1;
-            vpre2_[it] = tn2;
-            if (first_step) {
-                dt = t_j - t_j2;
-                dt0 = t_j - t_j1;
-                dt1 = t_j1 - t_j2;
-                if (scheme_ == 2) {
-                    mult = 1.0 / (t_j - t_j1);
-                    mult1 = mult;
-                    mult2 = 0.0;
-                }
-                first_step = false;
-            }
-            if (scheme_ == 3) {
-                mult = (dt + dt0) / (dt * dt0);
-                mult1 = dt / (dt1 * dt0);
-                mult2 = -dt0 / (dt * dt1);
-            }
-            if (!fcalc) factorize(mult);
-            launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, uj1, uj2, mult1, mult2,
-                       src_.n ? src_.p : nullptr, source_scale(it, -1.0), rhs.p);
-            solve(rhs.p, uj);
-            if (it >= D.nlst) keep_layer(it, uj);
-            if (tn1 < last_) output_layer(tn1, uj1);
-            if (tn == last_) output_layer(tn, uj);
-            // u_j2 <- u_j1, u_j1 <- u_j (pointer rotation; the old u_j2 buffer is reused)
-            double* tmp = uj2;
-            uj2 = uj1;
-            uj1 = uj;
-            uj = tmp;
-            fcalc = true;
-            ++steps_;
-            last_layer_ = it;
-        }
-        // layers kept for restarts that no later decade can need
-        for (auto itS = stored_.begin(); itS != stored_.end();) {
-            bool needed = false;
-            for (size_t k2 = k + 1; k2 < plan_.size(); ++k2)
-                if (itS->first == plan_[k2].npre1 || itS->first == plan_[k2].npre2) needed = true;
-            if (!needed && itS->first != last_layer_) itS = stored_.erase(itS);
-            else ++itS;
-        }
-    }
-    // the last field (a.last, A2/time_approx.cpp:1105-1111) is the newest u_j1
-    last_field_.alloc(vn);
-    EMTD_HIP(hipMemcpyAsync(last_field_.p, uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    EMTD_HIP(hipStreamSynchronize(stream_));
-    const double total = wall_seconds() - ts;
-    step_seconds_ = total - (factor_seconds_ - t_factor0);
-    ran_ = true;
+    bufA_.alloc(vn);
+    bufB_.alloc(vn);
+    bufC_.alloc(vn);
+    rhs_.alloc(vn);
+    L_ = LoopState();
+    L_.uj = bufA_.p;
+    L_.uj1 = bufB_.p;
+    L_.uj2 = bufC_.p;
+    vpre1_.assign(M_.times.size(), -1);
+    vpre2_.assign(M_.times.size(), -2);
+    begun_ = true;
 }
+
+// Sets up decade L_.k: initial field for the first decade, the two backward-Euler start
+// steps of the 3-point scheme (Two_Layers_Schema), or the restart of a later decade.
+void CalcCore::begin_decade() {
+    const std::vector<double>& t = M_.times;
+    const Decade& D = plan_[L_.k];
+    const size_t vn = size_t(n_) * npls_;
+    L_.fcalc = false;
+    L_.first_step = true;
+    int start;
+    if (1 >= D.tbeg && 1 < D.tend) {
+        for (double* p : {L_.uj, L_.uj1, L_.uj2})
+            EMTD_HIP(hipMemcpyAsync(p, u0_.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        L_.t_j = L_.t_j1 = t[0];
+        L_.tn = L_.tn1 = 0;
+        if (scheme_ == 3) {
+            L_.two_layer = 1;
+            start = 3;
+        } else {
+            start = 1;
+        }
+    } else {
+        start = D.tbeg;
+    }
+    L_.it = std::max(start, D.tbeg);
+    L_.in_decade = true;
+}
+
+void CalcCore::two_layer_step() {
+    // steps 1 and 2 with dt = t1 - t0 and no receiver output (A3/time_approx.cpp:575-790)
+    const std::vector<double>& t = M_.times;
+    const Decade& D = plan_[L_.k];
+    const size_t vn = size_t(n_) * npls_;
+    const int it = L_.two_layer;
+    const double dtl = t[1] - t[0];
+    if (it == 1) factorize(1.0 / dtl);
+    launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj, L_.uj, 1.0 / dtl, 0.0,
+               src_.n ? src_.p : nullptr, source_scale(it, dtl), rhs_.p);
+    solve(rhs_.p, L_.uj2);
+    std::swap(L_.uj, L_.uj2);
+    if (it >= D.nlst) keep_layer(it, L_.uj);
+    vpre1_[it] = it - 1;
+    vpre2_[it] = it - 2;
+    EMTD_HIP(hipMemcpyAsync(L_.uj2, L_.uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    EMTD_HIP(hipMemcpyAsync(L_.uj1, L_.uj, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    ++steps_;
+    last_layer_ = it;
+    if (it == 2 || it + 1 >= int(t.size())) {
+        L_.two_layer = 0;
+        L_.t_j2 = t[0];
+        L_.t_j1 = t[1];
+        L_.t_j = t[2];
+        L_.tn2 = 0;
+        L_.tn1 = 1;
+        L_.tn = 2;
+    } else {
+        L_.two_layer = it + 1;
+    }
+}
+
+void CalcCore::end_decade() {
+    // drop kept layers that no later decade restarts from
+    for (auto itS = stored_.begin(); itS != stored_.end();) {
+        bool needed = false;
+        for (size_t k2 = L_.k + 1; k2 < plan_.size(); ++k2)
+            if (itS->first == plan_[k2].npre1 || itS->first == plan_[k2].npre2) needed = true;
+        if (!needed) itS = stored_.erase(itS);
+        else ++itS;
+    }
+    L_.in_decade = false;
+    ++L_.k;
+    if (L_.k >= plan_.size()) {
+        const size_t vn = size_t(n_) * npls_;
+        // the last field (a.last, A2/time_approx.cpp:1105-1111) is the newest u_j1
+        last_field_.alloc(vn);
+        EMTD_HIP(hipMemcpyAsync(last_field_.p, L_.uj1, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        L_.done = true;
+        ran_ = true;
+    }
+}
+
+bool CalcCore::step() {
+    if (!begun_) begin();
+    if (L_.done) return false;
+    if (!L_.in_decade) begin_decade();
+    if (L_.two_layer) {
+        two_layer_step();
+        return true;
+    }
+    const std::vector<double>& t = M_.times;
+    const Decade& D = plan_[L_.k];
+    const size_t vn = size_t(n_) * npls_;
+    if (L_.it >= D.tend) {
+        end_decade();
+        return step();
+    }
+    const int it = L_.it;
+    if (stop_requested()) throw Error(EMTD_ERR_STOPPED, "stop file found");
+    bool fsdiff = false;
+    if (!L_.fcalc && it > (scheme_ == 2 ? 1 : 3)) {
+        auto a = stored_.find(D.npre1), b = stored_.find(D.npre2);
+        if (a == stored_.end() || b == stored_.end()) throw Error(EMTD_ERR_NOFILE, "restart layer missing");
+        EMTD_HIP(hipMemcpyAsync(L_.uj1, a->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        EMTD_HIP(hipMemcpyAsync(L_.uj2, b->second.p, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        fsdiff = true;
+    }
+    if (!fsdiff) {
+        L_.t_j2 = L_.t_j1; L_.t_j1 = L_.t_j; L_.t_j = t[it];
+        L_.tn2 = L_.tn1; L_.tn1 = L_.tn; L_.tn = it;
+    } else {
+        L_.t_j2 = t[D.npre2]; L_.t_j1 = t[D.npre1]; L_.t_j = t[it];
+        L_.tn2 = D.npre2; L_.tn1 = D.npre1; L_.tn = it;
+    }
+    vpre1_[it] = L_.tn1;
+    vpre2_[it] = L_.tn2;
+    if (L_.first_step) {
+        L_.dt = L_.t_j - L_.t_j2;
+        L_.dt0 = L_.t_j - L_.t_j1;
+        L_.dt1 = L_.t_j1 - L_.t_j2;
+        if (scheme_ == 2) {
+            L_.mult = 1.0 / (L_.t_j - L_.t_j1);
+            L_.mult1 = L_.mult;
+            L_.mult2 = 0.0;
+        }
+        L_.first_step = false;
+    }
+    if (scheme_ == 3) {
+        L_.mult = (L_.dt + L_.dt0) / (L_.dt * L_.dt0);
+        L_.mult1 = L_.dt / (L_.dt1 * L_.dt0);
+        L_.mult2 = -L_.dt0 / (L_.dt * L_.dt1);
+    }
+    if (!L_.fcalc) factorize(L_.mult);
+    launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj1, L_.uj2, L_.mult1, L_.mult2,
+               src_.n ? src_.p : nullptr, source_scale(it, -1.0), rhs_.p);
+    solve(rhs_.p, L_.uj);
+    if (it >= D.nlst) keep_layer(it, L_.uj);
+    if (L_.tn1 < last_) output_layer(L_.tn1, L_.uj1);
+    if (L_.tn == last_) output_layer(L_.tn, L_.uj);
+    // u_j2 <- u_j1, u_j1 <- u_j (pointer rotation; the old u_j2 buffer is reused)
+    double* tmp = L_.uj2;
+    L_.uj2 = L_.uj1;
+    L_.uj1 = L_.uj;
+    L_.uj = tmp;
+    L_.fcalc = true;
+    ++steps_;
+    last_layer_ = it;
+    ++L_.it;
+    if (L_.it >= D.tend) end_decade();
+    return true;
+}
+
+int CalcCore::advance(int nsteps) {
+    const double t0 = wall_seconds();
+    const double f0 = factor_seconds_;
+    int done = 0;
+    while ((nsteps < 0 || done < nsteps) && step()) ++done;
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    step_seconds_ += (wall_seconds() - t0) - (factor_seconds_ - f0);
+    return done;
+}
+
+void CalcCore::run() { advance(-1); }
 
 void CalcCore::receiver_values(int kind, double* out) const {
     if (!ran_) throw Error(EMTD_ERR_NOTINIT, "run before reading receivers");
@@ -469,6 +555,17 @@ void CalcCore::info(emtd_calc_info* inf) const {
     inf->factor_flops = S_->flops;
     inf->refine_iterations = refine_iterations_;
     inf->fallbacks = fallbacks_;
+    inf->factorizations = n_factorizations_;
+    inf->factor_gpu_seconds = factor_gpu_seconds_;
+    inf->solves = n_solves_;
+    inf->timed_solves = timed_solves_;
+    inf->solve_gpu_seconds = solve_gpu_seconds_;
+    inf->n_factored = nn_;
+    inf->n_fronts = S_->nf;
+    inf->n_levels = S_->nlevels;
+    inf->c_nnz = c_nnz_;
+    inf->nnz_low = nnz_;
+    inf->done = L_.done ? 1 : 0;
 }
 
 }  // namespace emtd
@@ -514,6 +611,23 @@ extern "C" int emtd_calc_run(emtd_calc* c) {
     CALC_GUARD_END
 }
 
+extern "C" int emtd_calc_step(emtd_calc* c, int nsteps, int* steps_done) {
+    CALC_GUARD_BEGIN
+    if (!c) throw Error(EMTD_ERR_RANGE, "null argument");
+    const int k = c->core->advance(nsteps);
+    if (steps_done) *steps_done = k;
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
+extern "C" int emtd_calc_set_timing(emtd_calc* c, int on) {
+    CALC_GUARD_BEGIN
+    if (!c) throw Error(EMTD_ERR_RANGE, "null argument");
+    c->core->set_timing(on != 0);
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
 extern "C" int emtd_calc_write_outputs(emtd_calc* c) {
     CALC_GUARD_BEGIN
     if (!c) throw Error(EMTD_ERR_RANGE, "null argument");
@@ -541,6 +655,7 @@ extern "C" int emtd_calc_last_field(const emtd_calc* c, double* out) {
 extern "C" int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info) {
     CALC_GUARD_BEGIN
     if (!c || !info) throw Error(EMTD_ERR_RANGE, "bad argument");
+    c->core->collect();
     c->core->info(info);
     return EMTD_OK;
     CALC_GUARD_END

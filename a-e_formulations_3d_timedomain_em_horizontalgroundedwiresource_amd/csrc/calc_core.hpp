@@ -22,12 +22,31 @@ class CalcCore {
     CalcCore(const std::string& dir, int form, int scheme, int precision, int device);
     ~CalcCore();
     void run();
+    int advance(int nsteps);  // runs up to nsteps steps (all when negative); synchronizes
+    bool step();
+    void set_timing(bool on);
+    void collect() { collect_solve_times(); }
     void receiver_values(int kind, double* out) const;
     void last_field(double* out) const;
     void write_outputs() const;
     void info(emtd_calc_info* inf) const;
 
   private:
+    struct LoopState {
+        size_t k = 0;
+        int it = 0;
+        bool in_decade = false, fcalc = false, first_step = true, done = false;
+        int two_layer = 0;
+        double mult = 0, mult1 = 0, mult2 = 0, dt = 0, dt0 = 0, dt1 = 0;
+        double t_j = 0, t_j1 = 0, t_j2 = 0;
+        int tn = 0, tn1 = 0, tn2 = 0;
+        double *uj = nullptr, *uj1 = nullptr, *uj2 = nullptr;
+    };
+    void begin();
+    void begin_decade();
+    void two_layer_step();
+    void end_decade();
+    void collect_solve_times();
     bool stop_requested() const;
     void factorize(double mult);
     void solve(const double* b, double* x);
@@ -65,7 +84,14 @@ class CalcCore {
     std::vector<int> vpre1_, vpre2_;
     std::map<int, DevBuf<double>> stored_;
     DevBuf<double> last_field_;
-    bool ran_ = false;
+    bool ran_ = false, begun_ = false;
+    LoopState L_;
+    DevBuf<double> bufA_, bufB_, bufC_, rhs_;
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_;
+    size_t ev_used_ = 0;
+    double solve_gpu_seconds_ = 0, factor_gpu_seconds_ = 0;
+    int64_t n_solves_ = 0, timed_solves_ = 0;
     double setup_seconds_ = 0, factor_seconds_ = 0, step_seconds_ = 0;
     int64_t steps_ = 0, n_factorizations_ = 0, refine_iterations_ = 0, fallbacks_ = 0;
 };

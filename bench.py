@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the implicit time step at ~1M DOF on one MI355X.
+
+Workload (default): the 80x80x50-cell A-formulation 2-point model (989,010 edge DOFs, 64^3
+log-normal conductivity field, seed 3; 200 steps from 0 to 1e-2 s in 10 decades with dt
+doubling). A "step" is one implicit time step exactly as the reference's decade loop does it:
+RHS (mass-matrix SpMV + source), forward/backward triangular sweeps with the GPU factor,
+receiver evaluation; the first step of each decade also refactorizes the system matrix.
+
+Prints ONE JSON line. ``value`` = timed steps / wall time of the timed region (inputs and
+factor storage already resident in HBM). The roofline object covers the dominant per-step
+kernels (the triangular sweeps): algorithmic bytes = 8 bytes x nnz(L) read by each of the
+two sweeps, per right-hand side, divided by their GPU time measured with HIP events.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = "a-e_formulations_3d_timedomain_em_horizontalgroundedwiresource_amd"
+
+METRIC = "timesteps/s at 1M DOF, FP32+IR vs FP64 at ≤1e-5 rel field err; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(cfg: int, cells=None, steps=None):
+    mg = importlib.import_module(PKG + ".modelgen")
+    spec = mg.config_spec(cfg, small=cells)
+    if steps is not None:
+        spec.times = mg.time_grid(1e-2, steps[0], steps[1])
+        tq = mg.quantize_times(spec.times)
+        spec.print_window = (tq[1], tq[-1])
+    d = tempfile.mkdtemp(prefix=f"emtd_cfg{cfg}_")
+    info = mg.write_model(spec, d)
+    return d, spec, info
+
+
+def cpu_baseline(cells=(20, 20, 16), steps_per_decade=20):
+    """The oracle (SuperLU, one core) on a bounded sample of the same workload family."""
+    mg = importlib.import_module(PKG + ".modelgen")
+    from oracle import calc, refio
+    spec = mg.config_spec(3, small=cells)
+    spec.times = mg.time_grid(1e-2 / 2 ** 9, steps_per_decade, 1)   # the first decade only
+    tq = mg.quantize_times(spec.times)
+    spec.print_window = (tq[1], tq[-1])
+    d = tempfile.mkdtemp(prefix="emtd_cpu_")
+    try:
+        info = mg.write_model(spec, d)
+        m = refio.read_model(d)
+        t0 = time.time()
+        calc.run(m, "A", 2)
+        dt = time.time() - t0
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    nsteps = len(spec.times) - 1
+    return {"value": nsteps / dt, "unit": "timesteps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (numpy/scipy SuperLU, 1 core) on the same model family at "
+                       f"{cells[0]}x{cells[1]}x{cells[2]} cells ({info['n']} edges), one decade: "
+                       f"assembly + 1 factorization + {nsteps} steps + receivers, {dt:.1f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=180)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--cells", type=str, default=None, help="override cells, e.g. 40x40x30")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              f"the multi-GPU design is described in DESIGN.md and not measured.", file=sys.stderr)
+        sys.exit(2)
+    emtd = importlib.import_module(PKG)
+    cells = tuple(int(v) for v in args.cells.split("x")) if args.cells else None
+    log(f"[bench] generating config {args.config} model {cells or ''}")
+    t0 = time.time()
+    d, spec, info = make_workload(args.config, cells)
+    log(f"[bench] model: {info} in {time.time() - t0:.1f} s")
+    prec = emtd.FP64 if args.precision == "fp64" else emtd.FP32_IR
+    try:
+        t0 = time.time()
+        c = emtd.Calc(d, 0 if spec.formulation == "A" else 1, spec.scheme, prec)
+        ci = c.info()
+        log(f"[bench] setup {time.time() - t0:.1f} s: n={ci.n_edges} factored={ci.n_factored} "
+            f"fronts={ci.n_fronts} levels={ci.n_levels} nnzL={ci.nnz_l:.4g} flops={ci.factor_flops:.4g}")
+        total_steps = ci.ntime - 1
+        W = min(args.warmup, total_steps - 1)
+        K = min(args.steps, total_steps - W)
+        log(f"[bench] warmup {W} steps")
+        c.step(W)
+        inf0 = c.info()
+        c.set_timing(True)
+        log(f"[bench] timed {K} steps")
+        t1 = time.perf_counter()
+        got = c.step(K)
+        t2 = time.perf_counter()
+        c.set_timing(False)
+        inf1 = c.info()
+        assert got == K, (got, K)
+        elapsed = t2 - t1
+        value = K / elapsed
+        nsolve = inf1.timed_solves
+        solve_s = inf1.solve_gpu_seconds / max(nsolve, 1)
+        elem = 8 if args.precision == "fp64" else 4
+        bytes_solve = 2 * elem * ci.nnz_l * ci.npls
+        achieved = bytes_solve / solve_s / 1e9
+        nfac = inf1.factorizations - inf0.factorizations
+        fac_gpu = (inf1.factor_gpu_seconds - inf0.factor_gpu_seconds) / max(nfac, 1)
+        fac_tflops = ci.factor_flops / fac_gpu / 1e12 if nfac else None
+        log(f"[bench] {K} steps in {elapsed:.3f} s -> {value:.2f} steps/s; solve {solve_s * 1e3:.3f} ms "
+            f"({achieved:.0f} GB/s); {nfac} factorizations {fac_gpu * 1e3:.1f} ms each "
+            f"({fac_tflops} TF/s)")
+        out = {
+            "metric": METRIC, "value": round(value, 4), "unit": "timesteps/s", "n_gpus": 1, "steps": K,
+            "warmup": W, "ms_per_step": round(1e3 * elapsed / K, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": (f"config {args.config}: {spec.formulation}-formulation {spec.scheme}-point, "
+                                    f"{len(spec.xs) - 1}x{len(spec.ys) - 1}x{len(spec.zs) - 1} cells, "
+                                    f"{ci.n_edges} edge DOFs, {total_steps} steps in {ci.n_decades} decades"),
+                       "n_edges": ci.n_edges, "nnz_L": ci.nnz_l, "factor_flops": ci.factor_flops,
+                       "precision": args.precision, "npls": ci.npls,
+                       "timed_factorizations": nfac, "factor_ms": round(fac_gpu * 1e3, 3),
+                       "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
+                       "setup_s": round(ci.setup_seconds, 2)},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "triangular sweeps k_fwd+k_bwd, one solve per step",
+                         "bytes_per_launch": bytes_solve},
+        }
+        c.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

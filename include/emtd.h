@@ -108,6 +108,13 @@ typedef struct emtd_calc emtd_calc;
 int emtd_calc_create(const char* workdir, int formulation, int scheme, int precision, int device,
                      emtd_calc** out);
 int emtd_calc_run(emtd_calc* c);             /* all decades, receivers recorded on device */
+/* Advances the time loop by up to nsteps implicit steps (all remaining when negative),
+ * refactorizing at decade starts as the reference does; returns after the GPU work is done.
+ * No counterpart in the reference (its loop is not resumable); used by the bench and the
+ * request loop. */
+int emtd_calc_step(emtd_calc* c, int nsteps, int* steps_done);
+/* Times every triangular solve with HIP events from now on (on != 0). */
+int emtd_calc_set_timing(emtd_calc* c, int on);
 int emtd_calc_write_outputs(emtd_calc* c);   /* Gather_edsall + tables into workdir */
 /* Receiver tables after emtd_calc_run: kind 0 = B receivers (curl of U), 1 = E receivers (U);
  * out holds (t_last-t_first+1) x npls x nrec x 3 values, undifferentiated. */
@@ -124,6 +131,16 @@ typedef struct emtd_calc_info {
     int64_t steps;
     double nnz_l, factor_flops;
     int64_t refine_iterations, fallbacks;
+    int64_t factorizations;      /* numeric factorizations so far */
+    double factor_gpu_seconds;   /* GPU time of those factorizations (HIP events) */
+    int64_t solves;              /* triangular solves (forward + backward sweep pairs) */
+    int64_t timed_solves;        /* solves timed with HIP events (see emtd_calc_set_timing) */
+    double solve_gpu_seconds;    /* GPU time of the timed solves */
+    int64_t n_factored;          /* DOFs in the factor (Dirichlet edges excluded) */
+    int64_t n_fronts, n_levels;  /* fronts and levels of the assembly tree */
+    int64_t c_nnz;               /* stored entries of the permuted mass matrix used by the RHS */
+    int64_t nnz_low;             /* strictly lower entries of the RSF pattern */
+    int done;                    /* 1 when every time layer has been computed */
 } emtd_calc_info;
 int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
 void emtd_calc_destroy(emtd_calc* c);

@@ -1,0 +1,62 @@
+"""The C-ABI library loads, exports every declared entry point, and its host-only parts
+(portrait) agree with the oracle. No GPU needed."""
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT, make_model
+from oracle import fem, refio
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "emtd.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(emtd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(emtd):
+    lib = emtd.lib()
+    decl = _declared()
+    assert decl, "no declarations parsed"
+    missing = [s for s in decl if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(emtd.EXPORTS) <= set(decl)
+
+
+def test_portrait_abi_matches_oracle(emtd, tmp_path):
+    d, _ = make_model(tmp_path, cfg=1, cells=(5, 4, 3))
+    m = refio.read_model(d)
+    mesh = emtd.Mesh(m.xyz, m.nver, m.ed, m.edges, m.nvkat, m.sigma, m.mu)
+    ig, jg = emtd.portrait(mesh)
+    rig, rjg = fem.portrait(m.ed, m.edges.shape[0])
+    np.testing.assert_array_equal(ig, rig)
+    np.testing.assert_array_equal(jg, rjg)
+
+
+def test_error_codes_without_device(emtd):
+    """Argument errors are reported before any device work."""
+    import ctypes as C
+    rc = emtd.lib().emtd_portrait(None, None, None, None)
+    assert rc == 0x0004
+    assert b"null" in emtd.lib().emtd_last_error()
+    out = C.c_void_p()
+    rc = emtd.lib().emtd_solver_create(C.c_int(-1), C.c_int(7), C.byref(out))
+    assert rc == 0x0004
+
+
+def test_modelgen_writes_reference_formats(tmp_path, mg):
+    d, spec = make_model(tmp_path, cfg=1, cells=(4, 3, 2))
+    nxn, nyn, nzn = len(spec.xs), len(spec.ys), len(spec.zs)
+    kpar = (nxn - 1) * (nyn - 1) * (nzn - 1)
+    nv = np.fromfile(os.path.join(d, "nver.dat"), dtype="<i4").reshape(kpar, 14)
+    assert nv[:, 8:].max() == 0 and nv[:, :8].min() == 1
+    e25 = np.fromfile(os.path.join(d, "edges.dat"), dtype="<i4").reshape(kpar, 25)
+    assert (e25[:, 12:24] == 0).all() and (e25[:, 24] == 1).all()
+    with open(os.path.join(d, "infite.0")) as f:
+        lines = f.read().splitlines()
+    assert lines[3].strip() == "T I M E :"
+    assert re.match(r"^ \d\.\d{7}e[+-]\d\d;", lines[4])
+    t = refio.read_infite0(os.path.join(d, "infite.0"))
+    assert len(t) == len(spec.times) and t[0] == 0.0
