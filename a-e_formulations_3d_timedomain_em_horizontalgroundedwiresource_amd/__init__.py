@@ -56,7 +56,8 @@ class CalcInfo(C.Structure):
                 ("fallbacks", C.c_int64), ("factorizations", C.c_int64), ("factor_gpu_seconds", C.c_double),
                 ("solves", C.c_int64), ("timed_solves", C.c_int64), ("solve_gpu_seconds", C.c_double),
                 ("n_factored", C.c_int64), ("n_fronts", C.c_int64), ("n_levels", C.c_int64),
-                ("c_nnz", C.c_int64), ("nnz_low", C.c_int64), ("done", C.c_int)]
+                ("c_nnz", C.c_int64), ("nnz_low", C.c_int64), ("done", C.c_int),
+                ("last_residual", C.c_double)]
 
 
 EXPORTS = [

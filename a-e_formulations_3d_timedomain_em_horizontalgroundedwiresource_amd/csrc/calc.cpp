@@ -74,6 +74,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     else
         c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
     perm_ = to_device(S_->perm, stream_);
+    if (precision_ == EMTD_FP32_IR) build_residual_operator();
     // C in permuted full CSR (exact zeros dropped) for the RHS
     {
         std::vector<double> cg(nnz_), cd(n_);
@@ -190,23 +191,80 @@ CalcCore::~CalcCore() {
 
 bool CalcCore::stop_requested() const { return file_exists(join_path(dir_, "stop")); }
 
+void CalcCore::build_residual_operator() {
+    // B and C on the factored rows/columns, permuted full CSR on the union pattern
+    std::vector<double> bg(nnz_), bd(n_), cg(nnz_), cd(n_);
+    b_ggl_.download(bg.data(), nnz_, stream_);
+    b_di_.download(bd.data(), n_, stream_);
+    c_ggl_.download(cg.data(), nnz_, stream_);
+    c_di_.download(cd.data(), n_, stream_);
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    std::vector<std::vector<std::pair<int, std::pair<double, double>>>> rows(nn_);
+    for (int i = 0; i < n_; ++i) {
+        const int a = S_->iperm[i];
+        if (a >= nn_) continue;
+        for (int k = ig_[i]; k < ig_[i + 1]; ++k) {
+            const int b = S_->iperm[jg_[k]];
+            if (b >= nn_) continue;
+            rows[a].push_back({b, {bg[k], cg[k]}});
+            rows[b].push_back({a, {bg[k], cg[k]}});
+        }
+    }
+    std::vector<int64_t> rp(nn_ + 1, 0);
+    std::vector<int> ci;
+    std::vector<double> vb, vc, db(nn_), dc(nn_);
+    for (int a = 0; a < nn_; ++a) {
+        std::sort(rows[a].begin(), rows[a].end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (auto& e : rows[a]) { ci.push_back(e.first); vb.push_back(e.second.first); vc.push_back(e.second.second); }
+        rp[a + 1] = int64_t(ci.size());
+        db[a] = bd[S_->perm[a]];
+        dc[a] = cd[S_->perm[a]];
+    }
+    krp_ = to_device(rp, stream_);
+    kci_ = to_device(ci, stream_);
+    kvb_ = to_device(vb, stream_);
+    kvc_ = to_device(vc, stream_);
+    kdb_ = to_device(db, stream_);
+    kdc_ = to_device(dc, stream_);
+    k_nnz_ = int64_t(ci.size());
+    norms_.alloc(size_t(2) * npls_);
+    h_norms_.assign(size_t(2) * npls_, 0.0);
+    if (const char* e = std::getenv("EMTD_IR_TOL")) ir_tol_ = std::atof(e);
+}
+
 void CalcCore::factorize(double mult) {
     const double t0 = wall_seconds();
-    int bad;
-    if (c64_) {
-        c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
-        bad = c64_->check_info();
-    } else {
+    cur_mult_ = mult;
+    int bad = -1;
+    use64_ = (precision_ == EMTD_FP64);
+    if (!use64_) {
         c32_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
         bad = c32_->check_info();
+        factor_gpu_seconds_ += c32_->last_factor_seconds();
+        if (bad >= 0) {  // FP32 pivot failure: this decade runs on an FP64 factor
+            use64_ = true;
+            ++fallbacks_;
+        }
+    }
+    if (use64_) {
+        if (!c64_) c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
+        c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
+        bad = c64_->check_info();
+        factor_gpu_seconds_ += c64_->last_factor_seconds();
     }
     if (bad >= 0) throw Error(EMTD_ERR_NOTSPD, "non-positive pivot at permuted index " + std::to_string(bad));
     factor_seconds_ += wall_seconds() - t0;
-    factor_gpu_seconds_ += c64_ ? c64_->last_factor_seconds() : c32_->last_factor_seconds();
     ++n_factorizations_;
 }
 
-void CalcCore::solve(const double* b, double* x) {
+void CalcCore::raw_solve(const double* b, double* x) {
+    if (use64_)
+        c64_->solve(b, x, npls_, n_);
+    else
+        c32_->solve(b, x, npls_, n_);
+}
+
+void CalcCore::solve(const double* b, double* x, const double* predictor) {
     if (timing_) {
         if (ev_used_ + 2 > ev_.size()) {
             hipEvent_t e;
@@ -214,16 +272,58 @@ void CalcCore::solve(const double* b, double* x) {
         }
         EMTD_HIP(hipEventRecord(ev_[ev_used_], stream_));
     }
-    if (c64_)
-        c64_->solve(b, x, npls_, n_);
-    else
-        c32_->solve(b, x, npls_, n_);
+    if (use64_) {
+        raw_solve(b, x);
+    } else {
+        refine(b, x, predictor);
+    }
     if (timing_) {
         EMTD_HIP(hipEventRecord(ev_[ev_used_ + 1], stream_));
         ev_used_ += 2;
     }
     launch_zero_tail(stream_, x, nn_, n_, npls_);
     ++n_solves_;
+}
+
+// FP64 iterative refinement around the FP32 factor: x = predictor, then
+// x += K32^{-1} (b - K x) until ||b - K x|| <= tol ||b|| for every source. A stalled
+// refinement refactors the decade in FP64 and solves the step directly.
+void CalcCore::refine(const double* b, double* x, const double* predictor) {
+    const size_t vn = size_t(n_) * npls_;
+    if (!ir_r_.n) { ir_r_.alloc(vn); ir_d_.alloc(vn); }
+    if (predictor && predictor != x)
+        EMTD_HIP(hipMemcpyAsync(x, predictor, vn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    else if (!predictor)
+        EMTD_HIP(hipMemsetAsync(x, 0, vn * sizeof(double), stream_));
+    double prev = 1e300;
+    for (int it = 0; it < ir_max_; ++it) {
+        launch_residual(stream_, nn_, n_, npls_, krp_.p, kci_.p, kvb_.p, kvc_.p, kdb_.p, kdc_.p, cur_mult_, b, x,
+                        ir_r_.p, norms_.p);
+        norms_.download(h_norms_.data(), h_norms_.size(), stream_);
+        EMTD_HIP(hipStreamSynchronize(stream_));
+        double worst = 0;
+        for (int q = 0; q < npls_; ++q) {
+            const double bn = std::sqrt(h_norms_[2 * q + 1]);
+            const double rn = std::sqrt(h_norms_[2 * q]);
+            worst = std::max(worst, bn > 0 ? rn / bn : rn);
+        }
+        last_residual_ = worst;
+        if (worst <= ir_tol_) return;
+        if (it > 0 && worst > 0.5 * prev) break;  // stalled
+        prev = worst;
+        c32_->solve(ir_r_.p, ir_d_.p, npls_, n_);
+        launch_axpy_rows(stream_, nn_, n_, npls_, ir_d_.p, x);
+        ++refine_iterations_;
+    }
+    // fallback: FP64 factor for the rest of this decade
+    ++fallbacks_;
+    use64_ = true;
+    if (!c64_) c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
+    c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, cur_mult_);
+    if (c64_->check_info() >= 0) throw Error(EMTD_ERR_NOTSPD, "FP64 fallback factorization failed");
+    factor_gpu_seconds_ += c64_->last_factor_seconds();
+    ++n_factorizations_;
+    c64_->solve(b, x, npls_, n_);
 }
 
 void CalcCore::set_timing(bool on) {
@@ -257,7 +357,8 @@ double CalcCore::source_scale(int itime, double dt_two_layer) const {
         if (cur[itime] == 0.0) {
             djdt = (cur[itime] - cur[itime - 1]) / dtb;
         } else {
-            // Time_approx_for_vfem::dA_dt called with (t, I[i-1], I[i], I[i+1], ...), literally
+            // Time_approx_for_vfem::dA_dt called with (t, I[i-1], I[i], I[i+1], ...), literally;
+            // at itime == 1 the reference reads time[-1] (out of bounds), defined here as time[0]
             double tt = t[itime];
             const double uj = cur[itime - 1], uj1 = cur[itime], uj2 = cur[itime + 1];
             const double dt = t[itime + 1] - t[itime - 1], dt0 = t[itime + 1] - t[itime], dt1 = t[itime] - t[itime - 1];
@@ -339,7 +440,7 @@ void CalcCore::two_layer_step() {
     if (it == 1) factorize(1.0 / dtl);
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj, L_.uj, 1.0 / dtl, 0.0,
                src_.n ? src_.p : nullptr, source_scale(it, dtl), rhs_.p);
-    solve(rhs_.p, L_.uj2);
+    solve(rhs_.p, L_.uj2, L_.uj);
     std::swap(L_.uj, L_.uj2);
     if (it >= D.nlst) keep_layer(it, L_.uj);
     vpre1_[it] = it - 1;
@@ -435,7 +536,7 @@ bool CalcCore::step() {
     if (!L_.fcalc) factorize(L_.mult);
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj1, L_.uj2, L_.mult1, L_.mult2,
                src_.n ? src_.p : nullptr, source_scale(it, -1.0), rhs_.p);
-    solve(rhs_.p, L_.uj);
+    solve(rhs_.p, L_.uj, L_.uj1);
     if (it >= D.nlst) keep_layer(it, L_.uj);
     if (L_.tn1 < last_) output_layer(L_.tn1, L_.uj1);
     if (L_.tn == last_) output_layer(L_.tn, L_.uj);
@@ -458,6 +559,8 @@ int CalcCore::advance(int nsteps) {
     int done = 0;
     while ((nsteps < 0 || done < nsteps) && step()) ++done;
     EMTD_HIP(hipStreamSynchronize(stream_));
+    if ((c64_ && c64_->sweep_error() != 0) || (c32_ && c32_->sweep_error() != 0))
+        throw Error(EMTD_ERR_ERROR, "triangular sweep wait limit reached");
     step_seconds_ += (wall_seconds() - t0) - (factor_seconds_ - f0);
     return done;
 }
@@ -555,6 +658,7 @@ void CalcCore::info(emtd_calc_info* inf) const {
     inf->factor_flops = S_->flops;
     inf->refine_iterations = refine_iterations_;
     inf->fallbacks = fallbacks_;
+    inf->last_residual = last_residual_;
     inf->factorizations = n_factorizations_;
     inf->factor_gpu_seconds = factor_gpu_seconds_;
     inf->solves = n_solves_;

@@ -49,7 +49,10 @@ class CalcCore {
     void collect_solve_times();
     bool stop_requested() const;
     void factorize(double mult);
-    void solve(const double* b, double* x);
+    void solve(const double* b, double* x, const double* predictor);
+    void raw_solve(const double* b, double* x);
+    void refine(const double* b, double* x, const double* predictor);
+    void build_residual_operator();
     double source_scale(int itime, double dt_two_layer) const;
     void output_layer(int layer, const double* u);
     double* keep_layer(int layer, const double* u);
@@ -92,6 +95,14 @@ class CalcCore {
     size_t ev_used_ = 0;
     double solve_gpu_seconds_ = 0, factor_gpu_seconds_ = 0;
     int64_t n_solves_ = 0, timed_solves_ = 0;
+    bool use64_ = true;
+    double cur_mult_ = 0, ir_tol_ = 1e-10, last_residual_ = 0;
+    int ir_max_ = 12;
+    DevBuf<int64_t> krp_;
+    DevBuf<int> kci_;
+    DevBuf<double> kvb_, kvc_, kdb_, kdc_, norms_, ir_r_, ir_d_;
+    std::vector<double> h_norms_;
+    int64_t k_nnz_ = 0;
     double setup_seconds_ = 0, factor_seconds_ = 0, step_seconds_ = 0;
     int64_t steps_ = 0, n_factorizations_ = 0, refine_iterations_ = 0, fallbacks_ = 0;
 };

@@ -182,12 +182,16 @@ __global__ void k_trsm(const DevFront* __restrict__ F, const int* __restrict__ f
 
 template <typename T>
 struct Mfma;
+// Operands of both instructions: lane l carries A[l%16][l/16] and B[l/16][l%16]. The result
+// layouts differ (checked on gfx950 with an in-bounds probe): accumulator element v of lane l
+// is D[row(l/16, v)][l%16] with row = l/16 + 4v for f64 and 4(l/16) + v for f32.
 template <>
 struct Mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -195,61 +199,71 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
-// Trailing update of the lower frontal matrix: F[r][c] -= sum_k P[r][k] P[c][k] for
-// r >= c >= kb+nb, in 64x64 tiles of 4 wavefronts; each wavefront owns a 32x32 quarter as
-// 2x2 MFMA 16x16x4 blocks (A lane l: row l%16, k l/16; C lane l: rows 4(l/16)..+3, col l%16).
+// Symmetric rank-k update of the lower frontal matrix with the factored columns
+// [k0, k0+kd), kd = min(kmax, p-k0):  F[r][c] -= sum_k L[r][k] L[c][k]  for r >= c in the
+// target columns [k0+kd, ce), where ce = min(kb+NBO, p) (mode 0: update inside the current
+// 128-column block) or m (mode 1: the trailing matrix). 64x64 tiles (blockIdx.x = row tile,
+// blockIdx.y = column tile, blockIdx.z = front); 4 wavefronts own a 32x32 quarter each as
+// 2x2 MFMA 16x16x4 blocks (lane layouts in Mfma<T>);
+// K is streamed through LDS in chunks of 32 and the tile is read and written once.
+constexpr int NBO = 128;  // outer block of the factorization
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                int kb, T* __restrict__ L, T* __restrict__ U) {
-    const DevFront Fr = F[flist[blockIdx.y]];
-    if (kb >= Fr.p) return;
-    const int nb = min(NB, Fr.p - kb);
-    const int base = kb + nb;
-    const int rem = Fr.m - base;
-    if (rem <= 0) return;
-    const int nt = (rem + TILE - 1) / TILE;
-    const int t = blockIdx.x;
-    if (t >= nt * (nt + 1) / 2) return;
-    int ti = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while (ti * (ti + 1) / 2 > t) --ti;
-    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int R0 = base + TILE * ti, C0 = base + TILE * tj;
-    constexpr int KP = NB + 4;  // padded K extent (multiple of 4) + bank offset
+__global__ void __launch_bounds__(256) k_syrk(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
+                                              int k0, int kmax, int mode, T* __restrict__ L, T* __restrict__ U) {
+    const DevFront Fr = F[flist[blockIdx.z]];
+    const int kd = min(kmax, Fr.p - k0);
+    if (kd <= 0) return;
+    const int cs = k0 + kd;
+    const int ce = mode == 0 ? min(kb + NBO, Fr.p) : Fr.m;
+    if (cs >= ce) return;
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (ti < tj) return;
+    const int R0 = cs + TILE * ti, C0 = cs + TILE * tj;
+    if (R0 >= Fr.m || C0 >= ce) return;
+    constexpr int KC = 32, KP = KC + 4;
     __shared__ T As[TILE][KP];
     __shared__ T Bs[TILE][KP];
-    const T* pan = L + Fr.loff + int64_t(kb) * Fr.m;
-    const int nb4 = (nb + 3) & ~3;
-    for (int idx = threadIdx.x; idx < TILE * nb4; idx += blockDim.x) {
-        const int i = idx % TILE, k = idx / TILE;
-        const int ra = R0 + i, rb = C0 + i;
-        As[i][k] = (k < nb && ra < Fr.m) ? pan[int64_t(k) * Fr.m + ra] : T(0);
-        Bs[i][k] = (k < nb && rb < Fr.m) ? pan[int64_t(k) * Fr.m + rb] : T(0);
-    }
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    if (ti == tj && wr + 31 < wc) return;  // strictly upper quarter of a diagonal tile
+    const bool skip = (ti == tj && wr + 31 < wc);  // strictly upper quarter of a diagonal tile
     typename Mfma<T>::acc_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
     const int li = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < nb4; k0 += 4) {
-        T av[2], bv[2];
-        for (int a = 0; a < 2; ++a) av[a] = As[wr + 16 * a + li][k0 + lk];
-        for (int b = 0; b < 2; ++b) bv[b] = Bs[wc + 16 * b + li][k0 + lk];
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
+    for (int kc = 0; kc < kd; kc += KC) {
+        const int nk = min(KC, kd - kc);
+        const T* pan = L + Fr.loff + int64_t(k0 + kc) * Fr.m;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < TILE * KC; idx += blockDim.x) {
+            const int i = idx % TILE, k = idx / TILE;
+            const int ra = R0 + i, rb = C0 + i;
+            As[i][k] = (k < nk && ra < Fr.m) ? pan[int64_t(k) * Fr.m + ra] : T(0);
+            Bs[i][k] = (k < nk && rb < ce) ? pan[int64_t(k) * Fr.m + rb] : T(0);
+        }
+        __syncthreads();
+        if (!skip) {
+            const int nk4 = (nk + 3) & ~3;
+            for (int kk = 0; kk < nk4; kk += 4) {
+                T av[2], bv[2];
+                for (int a = 0; a < 2; ++a) av[a] = As[wr + 16 * a + li][kk + lk];
+                for (int b = 0; b < 2; ++b) bv[b] = Bs[wc + 16 * b + li][kk + lk];
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
+            }
+        }
     }
+    if (skip) return;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int v = 0; v < 4; ++v) {
-                const int r = R0 + wr + 16 * a + 4 * lk + v;
+                const int r = R0 + wr + 16 * a + Mfma<T>::row(lk, v);
                 const int c = C0 + wc + 16 * b + li;
-                if (r < Fr.m && c < Fr.m && r >= c) *front_elem(Fr, L, U, r, c) -= acc[a][b][v];
+                if (r < Fr.m && c < ce && r >= c) *front_elem(Fr, L, U, r, c) -= acc[a][b][v];
             }
 }
 
@@ -349,6 +363,213 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
     for (int i = threadIdx.x; i < p; i += blockDim.x) xx[Fr.fbeg + i] = v[i];
 }
 
+// ------------------------------------------------------------------ sweeps of large fronts
+// Levels with few, large fronts split every front over many workgroups: 64-row blocks in
+// the forward sweep, 64-column pivot blocks in the backward sweep. A workgroup takes a ticket
+// (atomic counter) and processes items in ticket order, so it only ever waits for blocks held
+// by workgroups that started earlier; results of a pivot block are published through a
+// per-block buffer and flag (release/acquire at agent scope). The wait is bounded: after
+// kSpinLimit polls the workgroup records an error and continues, so the kernel always ends.
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ void wait_flag(const int* flag, int* err) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kSpinLimit) {
+            atomicOr(err, 1);
+            break;
+        }
+    }
+}
+
+__device__ __forceinline__ void publish_flag(int* flag) {
+    __threadfence();
+    __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// v_f = [b_P ; 0] + children's update vectors (one workgroup per front)
+__global__ void __launch_bounds__(256) k_fwd_gather(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                    const int* __restrict__ children, const int* __restrict__ relmap,
+                                                    double* __restrict__ W, const double* __restrict__ b) {
+    const DevFront Fr = F[flist[blockIdx.x]];
+    double* v = W + Fr.woff;
+    for (int i = threadIdx.x; i < Fr.m; i += blockDim.x) v[i] = i < Fr.p ? b[Fr.fbeg + i] : 0.0;
+    __syncthreads();
+    for (int ci = Fr.child_beg; ci < Fr.child_end; ++ci) {
+        const DevFront Fc = F[children[ci]];
+        const int uc = Fc.m - Fc.p;
+        const int* rm = relmap + Fc.relmap_off;
+        const double* wc = W + Fc.woff + Fc.p;
+        for (int k = threadIdx.x; k < uc; k += blockDim.x) v[rm[k]] += wc[k];
+        __syncthreads();
+    }
+}
+
+// v_f = [y_P ; x_U] for the backward sweep (one workgroup per front)
+__global__ void __launch_bounds__(256) k_bwd_gather(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                    const int* __restrict__ rows, double* __restrict__ W,
+                                                    const double* __restrict__ x) {
+    const DevFront Fr = F[flist[blockIdx.x]];
+    double* v = W + Fr.woff;
+    const int* rw = rows + Fr.rows_off;
+    for (int i = threadIdx.x; i < Fr.m; i += blockDim.x) v[i] = i < Fr.p ? x[Fr.fbeg + i] : x[rw[i]];
+}
+
+// item.x = front, item.y = 64-row block; flag_off per front in foff[]
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_big(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                 const int* __restrict__ foff, int* __restrict__ flags,
+                                                 double* __restrict__ ybuf, int* __restrict__ counter,
+                                                 int* __restrict__ err, const T* __restrict__ L,
+                                                 double* __restrict__ W, double* __restrict__ x) {
+    __shared__ int s_ticket;
+    __shared__ double red[4][64];
+    __shared__ double ys[64];
+    if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
+    __syncthreads();
+    const int2 it = items[s_ticket];
+    const DevFront Fr = F[it.x];
+    const int rb = it.y;
+    const int p = Fr.p, m = Fr.m;
+    const int np = (p + 63) / 64;
+    const int r0 = 64 * rb;
+    const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = r0 + lr;
+    const T* Lf = L + Fr.loff;
+    double* v = W + Fr.woff;
+    int* fl = flags + foff[it.x];
+    double* yb = ybuf + int64_t(foff[it.x]) * 64;
+    const int ncb = (r0 < p) ? rb : np;
+    double acc = 0.0;
+    for (int cb = 0; cb < ncb; ++cb) {
+        if (threadIdx.x == 0) wait_flag(fl + cb, err);
+        __syncthreads();
+        const int nbc = min(64, p - 64 * cb);
+        double yv[16];
+        for (int j = 0; j < 16; ++j) {
+            const int jj = 16 * q + j;
+            yv[j] = jj < nbc ? __hip_atomic_load(yb + 64 * cb + jj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        }
+        if (r < m) {
+            const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
+            for (int j = 0; j < 16; ++j)
+                if (16 * q + j < nbc) acc += double(lc[int64_t(j) * m]) * yv[j];
+        }
+    }
+    red[q][lr] = acc;
+    __syncthreads();
+    double rowv = 0.0;
+    if (q == 0 && r < m) rowv = v[r] - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]);
+    if (r0 < p) {
+        const int nbp = min(64, p - r0);
+        if (q == 0) {
+            double val = rowv;
+            for (int j = 0; j < nbp; ++j) {
+                const double yj = __shfl(val, j, 64) / double(Lf[int64_t(r0 + j) * m + r0 + j]);
+                if (lr == j) ys[j] = yj;
+                if (lr > j && lr < nbp) val -= double(Lf[int64_t(r0 + j) * m + r0 + lr]) * yj;
+            }
+            if (lr < nbp) {
+                const double y = ys[lr];
+                __hip_atomic_store(yb + 64 * rb + lr, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v[r] = y;
+                x[Fr.fbeg + r] = y;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) publish_flag(fl + rb);
+        // update rows sharing this block: subtract the block's own contribution
+        if (q == 0 && lr >= nbp && r < m) {
+            double a2 = rowv;
+            for (int j = 0; j < nbp; ++j) a2 -= double(Lf[int64_t(r0 + j) * m + r]) * ys[j];
+            v[r] = a2;
+        }
+    } else if (q == 0 && r < m) {
+        v[r] = rowv;
+    }
+}
+
+// item.x = front, item.y = 64-column pivot block (items in decreasing block order per front)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_big(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                 const int* __restrict__ foff, int* __restrict__ flags,
+                                                 double* __restrict__ xbuf, int* __restrict__ counter,
+                                                 int* __restrict__ err, const T* __restrict__ L,
+                                                 double* __restrict__ W, double* __restrict__ x) {
+    __shared__ int s_ticket;
+    __shared__ double sred[64];
+    __shared__ double xs[64];
+    __shared__ double xr[64];
+    if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
+    __syncthreads();
+    const int2 it = items[s_ticket];
+    const DevFront Fr = F[it.x];
+    const int cb = it.y;
+    const int p = Fr.p, m = Fr.m;
+    const int np = (p + 63) / 64;
+    const int c0 = 64 * cb;
+    const int nbc = min(64, p - c0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const T* Lf = L + Fr.loff;
+    double* v = W + Fr.woff;
+    int* fl = flags + foff[it.x];
+    double* xb = xbuf + int64_t(foff[it.x]) * 64;
+    double part[16];
+    for (int j = 0; j < 16; ++j) part[j] = 0.0;
+    // update rows first (their values are final), in 64-row chunks
+    for (int rr = p; rr < m; rr += 64) {
+        const int r = rr + lane;
+        const double xv = r < m ? v[r] : 0.0;
+        if (r < m)
+            for (int j = 0; j < 16; ++j) {
+                const int jj = wave + 4 * j;
+                if (jj < nbc) part[j] += double(Lf[int64_t(c0 + jj) * m + r]) * xv;
+            }
+    }
+    // later pivot blocks, newest first
+    for (int cb2 = np - 1; cb2 > cb; --cb2) {
+        if (threadIdx.x == 0) wait_flag(fl + cb2, err);
+        __syncthreads();
+        const int nb2 = min(64, p - 64 * cb2);
+        if (threadIdx.x < 64)
+            xr[lane] = lane < nb2 ? __hip_atomic_load(xb + 64 * cb2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0.0;
+        __syncthreads();
+        const int r = 64 * cb2 + lane;
+        if (lane < nb2)
+            for (int j = 0; j < 16; ++j) {
+                const int jj = wave + 4 * j;
+                if (jj < nbc) part[j] += double(Lf[int64_t(c0 + jj) * m + r]) * xr[lane];
+            }
+        __syncthreads();
+    }
+    // rows below the diagonal inside the block itself are handled by the triangular solve
+    for (int j = 0; j < 16; ++j) {
+        double s = part[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        const int jj = wave + 4 * j;
+        if (lane == 0 && jj < 64) sred[jj] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double val = lane < nbc ? v[c0 + lane] - sred[lane] : 0.0;
+        for (int j = nbc - 1; j >= 0; --j) {
+            const double xj = __shfl(val, j, 64) / double(Lf[int64_t(c0 + j) * m + c0 + j]);
+            if (lane == j) xs[j] = xj;
+            if (lane < j) val -= double(Lf[int64_t(c0 + lane) * m + c0 + j]) * xj;
+        }
+        if (lane < nbc) {
+            const double xv = xs[lane];
+            __hip_atomic_store(xb + 64 * cb + lane, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v[c0 + lane] = xv;
+            x[Fr.fbeg + c0 + lane] = xv;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) publish_flag(fl + cb);
+}
+
 inline unsigned grid_for(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;
     return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20)));
@@ -426,22 +647,34 @@ void GpuCholesky<T>::numeric() {
         const std::vector<int>& fl = level_sorted_[l];
         const int* dfl = level_sorted_dev_.p + level_sorted_off_[l];
         const int maxp = S.pcount[fl[0]];
-        int maxm = 0;
-        for (int f : fl) maxm = std::max(maxm, S.mcount[f]);
         int nact = int(fl.size());
-        for (int kb = 0; kb < maxp; kb += NB) {
+        for (int kb = 0; kb < maxp; kb += NBO) {
             while (nact > 0 && S.pcount[fl[nact - 1]] <= kb) --nact;
             if (nact == 0) break;
             int mm = 0;
             for (int i = 0; i < nact; ++i) mm = std::max(mm, S.mcount[fl[i]]);
-            hipLaunchKernelGGL(k_potrf<T>, dim3(nact), dim3(256), 0, stream_, fronts_.p, dfl, kb, L_.p, info_.p);
-            const int rem = mm - kb - 1;
-            if (rem > 0) {
-                const int chunks = (rem + 63) / 64;
-                hipLaunchKernelGGL(k_trsm<T>, dim3(chunks, nact), dim3(64), 0, stream_, fronts_.p, dfl, kb, L_.p);
-                const int nt = (rem + TILE - 1) / TILE;
-                hipLaunchKernelGGL(k_update<T>, dim3(nt * (nt + 1) / 2, nact), dim3(256), 0, stream_, fronts_.p,
-                                   dfl, kb, L_.p, U_.p);
+            for (int nz = 0; nz < nact; nz += 65535) {
+                const int cnt = std::min(65535, nact - nz);
+                const int* fz = dfl + nz;
+                for (int ks = kb; ks < std::min(kb + NBO, maxp); ks += NB) {
+                    hipLaunchKernelGGL(k_potrf<T>, dim3(cnt), dim3(256), 0, stream_, fronts_.p, fz, ks, L_.p, info_.p);
+                    const int rem = mm - ks - 1;
+                    if (rem <= 0) continue;
+                    hipLaunchKernelGGL(k_trsm<T>, dim3((rem + 63) / 64, cnt), dim3(64), 0, stream_, fronts_.p, fz, ks,
+                                       L_.p);
+                    // inside the outer block: columns [ks+nb, kb+NBO)
+                    const int pc = (kb + NBO) - (ks + 1);
+                    if (pc > 0)
+                        hipLaunchKernelGGL(k_syrk<T>, dim3((rem + TILE - 1) / TILE, (pc + TILE - 1) / TILE, cnt),
+                                           dim3(256), 0, stream_, fronts_.p, fz, kb, ks, NB, 0, L_.p, U_.p);
+                }
+                // trailing matrix: columns [kb+nbo, m) with K = nbo
+                const int rem = mm - kb - 1;
+                if (rem > 0) {
+                    const int t = (rem + TILE - 1) / TILE;
+                    hipLaunchKernelGGL(k_syrk<T>, dim3(t, t, cnt), dim3(256), 0, stream_, fronts_.p, fz, kb, kb, NBO, 1,
+                                       L_.p, U_.p);
+                }
             }
         }
     }
@@ -512,22 +745,102 @@ int GpuCholesky<T>::check_info() {
 }
 
 template <typename T>
+void GpuCholesky<T>::prepare_big_levels() {
+    const Symbolic& S = S_;
+    big_.assign(S.nlevels, 0);
+    std::vector<int2> fi, bi;
+    std::vector<int> foff(S.nf, 0);
+    big_fwd_ptr_.assign(S.nlevels + 1, 0);
+    big_bwd_ptr_.assign(S.nlevels + 1, 0);
+    int64_t nflags = 0;
+    int maxflags = 0;
+    for (int l = 0; l < S.nlevels; ++l) {
+        const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
+        int maxp = 0;
+        for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) maxp = std::max(maxp, S.pcount[S.level_fronts[k]]);
+        big_[l] = (nfl < big_level_fronts_ && maxp > 64) ? 1 : 0;
+        if (big_[l]) {
+            int off = 0;
+            for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
+                const int f = S.level_fronts[k];
+                foff[f] = off;
+                const int np = (S.pcount[f] + 63) / 64, nr = (S.mcount[f] + 63) / 64;
+                for (int rb = 0; rb < nr; ++rb) fi.push_back(make_int2(f, rb));
+                for (int cb = np - 1; cb >= 0; --cb) bi.push_back(make_int2(f, cb));
+                off += np;
+            }
+            maxflags = std::max(maxflags, off);
+        }
+        big_fwd_ptr_[l + 1] = int(fi.size());
+        big_bwd_ptr_[l + 1] = int(bi.size());
+    }
+    (void)nflags;
+    big_fwd_items_ = to_device(fi.empty() ? std::vector<int2>{make_int2(0, 0)} : fi, stream_);
+    big_bwd_items_ = to_device(bi.empty() ? std::vector<int2>{make_int2(0, 0)} : bi, stream_);
+    foff_ = to_device(foff, stream_);
+    max_flags_ = std::max(maxflags, 1);
+    flags_.alloc(size_t(max_flags_) + 2);   // flags + ticket counter + error word
+    xbuf_.alloc(size_t(max_flags_) * 64);
+    std::vector<int> z(max_flags_ + 2, 0);
+    flags_.upload(z.data(), z.size(), stream_);
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    big_ready_ = true;
+}
+
+template <typename T>
+int GpuCholesky<T>::sweep_error() {
+    int h = 0;
+    if (!big_ready_ || !flags_.p) return 0;
+    EMTD_HIP(hipMemcpyAsync(&h, flags_.p + max_flags_ + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    return h;
+}
+
+template <typename T>
 void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld) {
     const Symbolic& S = S_;
+    if (!big_ready_) prepare_big_levels();
     if (nrhs > wrhs_) {
         W_.alloc(size_t(std::max<int64_t>(S.wsize, 1)) * nrhs);
         wrhs_ = nrhs;
     }
+    int* counter = flags_.p + max_flags_;
+    int* err = flags_.p + max_flags_ + 1;
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
-        hipLaunchKernelGGL(k_fwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p,
-                           level_fronts_.p + S.level_ptr[l], children_.p, relmap_.p, L_.p, W_.p, S.wsize, d_b, d_x,
-                           ld);
+        const int* lf = level_fronts_.p + S.level_ptr[l];
+        if (!big_[l]) {
+            hipLaunchKernelGGL(k_fwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p, lf, children_.p,
+                               relmap_.p, L_.p, W_.p, S.wsize, d_b, d_x, ld);
+            continue;
+        }
+        const int b = big_fwd_ptr_[l], e = big_fwd_ptr_[l + 1];
+        for (int r = 0; r < nrhs; ++r) {
+            hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
+                               int64_t(max_flags_ + 1), 0);
+            hipLaunchKernelGGL(k_fwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, children_.p, relmap_.p,
+                               W_.p + r * S.wsize, d_b + r * ld);
+            hipLaunchKernelGGL(k_fwd_big<T>, dim3(e - b), dim3(256), 0, stream_, fronts_.p, big_fwd_items_.p + b,
+                               foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, W_.p + r * S.wsize, d_x + r * ld);
+        }
     }
     for (int l = S.nlevels - 1; l >= 0; --l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
-        hipLaunchKernelGGL(k_bwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p,
-                           level_fronts_.p + S.level_ptr[l], rows_.p, L_.p, W_.p, S.wsize, d_x, ld);
+        const int* lf = level_fronts_.p + S.level_ptr[l];
+        if (!big_[l]) {
+            hipLaunchKernelGGL(k_bwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p, lf, rows_.p, L_.p, W_.p,
+                               S.wsize, d_x, ld);
+            continue;
+        }
+        const int b = big_bwd_ptr_[l], e = big_bwd_ptr_[l + 1];
+        for (int r = 0; r < nrhs; ++r) {
+            hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
+                               int64_t(max_flags_ + 1), 0);
+            hipLaunchKernelGGL(k_bwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, rows_.p,
+                               W_.p + r * S.wsize, d_x + r * ld);
+            hipLaunchKernelGGL(k_bwd_big<T>, dim3(e - b), dim3(256), 0, stream_, fronts_.p, big_bwd_items_.p + b,
+                               foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, W_.p + r * S.wsize, d_x + r * ld);
+        }
     }
     EMTD_HIP(hipGetLastError());
 }

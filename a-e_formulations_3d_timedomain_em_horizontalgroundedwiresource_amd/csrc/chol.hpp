@@ -41,6 +41,10 @@ class GpuCholesky {
 
     // Returns -1 or the first (permuted) pivot that was not positive; synchronizes.
     int check_info();
+    // Nonzero when a large-front sweep hit its wait limit (synchronizes).
+    int sweep_error();
+    // Levels with fewer fronts than this use the multi-workgroup sweeps.
+    void set_big_level_fronts(int k) { big_level_fronts_ = k; big_ready_ = false; }
     double last_factor_seconds() const { return factor_seconds_; }
 
     const Symbolic& sym() const { return S_; }
@@ -48,6 +52,7 @@ class GpuCholesky {
 
   private:
     void numeric();
+    void prepare_big_levels();
     const Symbolic& S_;
     hipStream_t stream_;
     DevBuf<T> L_, U_;
@@ -61,6 +66,14 @@ class GpuCholesky {
     std::vector<int> level_sorted_off_;
     DevBuf<int> info_;
     int wrhs_ = 0;
+    int big_level_fronts_ = 64;
+    bool big_ready_ = false;
+    std::vector<char> big_;
+    std::vector<int> big_fwd_ptr_, big_bwd_ptr_;
+    DevBuf<int2> big_fwd_items_, big_bwd_items_;
+    DevBuf<int> foff_, flags_;
+    DevBuf<double> xbuf_;
+    int max_flags_ = 1;
     double factor_seconds_ = 0;
 };
 

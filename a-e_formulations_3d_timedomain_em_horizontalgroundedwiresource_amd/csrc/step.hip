@@ -63,7 +63,73 @@ __global__ void k_receivers(int nrows, int npls, const int64_t* __restrict__ rp,
     if (lane == 0) out[int64_t(r) * nrows + row] = acc;
 }
 
+// r = b - K x with K = mult*C + B on the factored rows/columns (diagonal with the 1e-12 bump,
+// A2/time_approx.cpp:987-994); also accumulates ||r||^2 and ||b||^2 into norms[2*rhs..].
+__global__ void __launch_bounds__(256) k_residual(int nn, int64_t n, int npls, const int64_t* __restrict__ rp,
+                                                  const int* __restrict__ ci, const double* __restrict__ vb,
+                                                  const double* __restrict__ vc, const double* __restrict__ db,
+                                                  const double* __restrict__ dc, double mult,
+                                                  const double* __restrict__ b, const double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ norms) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sr[8], sb[8];
+    for (int q = 0; q < npls; ++q) {
+        double rr = 0.0, bb = 0.0;
+        if (i < nn) {
+            const double* xq = x + q * n;
+            double d = dc[i] * mult + db[i];
+            d += d * 1e-12;
+            double acc = d * xq[i];
+            for (int64_t k = rp[i]; k < rp[i + 1]; ++k) acc += (vc[k] * mult + vb[k]) * xq[ci[k]];
+            const double bi = b[i + q * n];
+            const double ri = bi - acc;
+            r[i + q * n] = ri;
+            rr = ri * ri;
+            bb = bi * bi;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            rr += __shfl_down(rr, o, 64);
+            bb += __shfl_down(bb, o, 64);
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { sr[w] = rr; sb[w] = bb; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0, c = 0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { a += sr[k]; c += sb[k]; }
+            atomicAdd(norms + 2 * q, a);
+            atomicAdd(norms + 2 * q + 1, c);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_axpy_rows(int nn, int64_t n, int npls, const double* __restrict__ d, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    for (int q = 0; q < npls; ++q) x[i + q * n] += d[i + q * n];
+}
+
+__global__ void k_zero(double* __restrict__ p, int cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = 0.0;
+}
+
 }  // namespace
+
+void launch_residual(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* vb,
+                     const double* vc, const double* db, const double* dc, double mult, const double* b,
+                     const double* x, double* r, double* norms) {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, norms, 2 * npls);
+    if (nn <= 0) return;
+    hipLaunchKernelGGL(k_residual, dim3((nn + 255) / 256), dim3(256), 0, s, nn, n, npls, rp, ci, vb, vc, db, dc, mult,
+                       b, x, r, norms);
+}
+
+void launch_axpy_rows(hipStream_t s, int nn, int64_t n, int npls, const double* d, double* x) {
+    if (nn <= 0) return;
+    hipLaunchKernelGGL(k_axpy_rows, dim3((nn + 255) / 256), dim3(256), 0, s, nn, n, npls, d, x);
+}
 
 void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
                 const double* u1, const double* u2, double m1, double m2, const double* src, double scale,

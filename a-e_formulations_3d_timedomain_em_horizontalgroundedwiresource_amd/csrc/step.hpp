@@ -10,6 +10,10 @@ namespace emtd {
 void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
                 const double* u1, const double* u2, double m1, double m2, const double* src, double scale,
                 double* b);
+void launch_residual(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* vb,
+                     const double* vc, const double* db, const double* dc, double mult, const double* b,
+                     const double* x, double* r, double* norms);
+void launch_axpy_rows(hipStream_t s, int nn, int64_t n, int npls, const double* d, double* x);
 void launch_zero_tail(hipStream_t s, double* x, int nn, int64_t n, int npls);
 void launch_receivers(hipStream_t s, int nrows, int npls, const int64_t* rp, const int* idx, const double* w,
                       const double* u, int64_t n, double* out);

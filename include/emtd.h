@@ -141,6 +141,7 @@ typedef struct emtd_calc_info {
     int64_t c_nnz;               /* stored entries of the permuted mass matrix used by the RHS */
     int64_t nnz_low;             /* strictly lower entries of the RSF pattern */
     int done;                    /* 1 when every time layer has been computed */
+    double last_residual;        /* FP32_IR: last relative residual ||b-Kx||/||b|| */
 } emtd_calc_info;
 int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
 void emtd_calc_destroy(emtd_calc* c);

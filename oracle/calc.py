@@ -183,9 +183,12 @@ def source_scale(model: Model, form: str, itime: int, dt_two_layer: float | None
         if cur[itime] == 0:
             djdt = (cur[itime] - cur[itime - 1]) / dt_back
         else:
+            # the reference reads time[itime-2] here, i.e. time[-1] (out of bounds, undefined)
+            # at itime == 1; the restatement defines it as time[0]
+            tj2 = t[itime - 2] if itime >= 2 else t[0]
             djdt = _dA_dt(t[itime], cur[itime - 1], cur[itime], cur[itime + 1],
                           t[itime + 1] - t[itime - 1], t[itime + 1] - t[itime], t[itime] - t[itime - 1],
-                          t[itime], t[itime - 1], t[itime - 2])
+                          t[itime], t[itime - 1], tj2)
     else:
         djdt = (cur[itime] - cur[itime - 1]) / dt_back
     return -djdt

@@ -46,6 +46,8 @@ CASES = [
     ("E", 2, None),
     ("E", 3, None),
     ("A", 2, _ramp),
+    ("A", 3, _ramp),
+    ("E", 2, _ramp),
     ("E", 3, _ramp),
 ]
 
@@ -94,3 +96,28 @@ def test_calc_drop_in_executable(emtd, tmp_path):
     assert r.returncode == 0, r.stderr
     for stem in ("eall_anom.1", "edsall_anom.1", "ball_anom.1", "a.last", "DecIg", "nDec"):
         assert os.path.exists(os.path.join(d, stem)), stem
+
+
+TOL32 = 1e-5
+
+
+@pytest.mark.parametrize("form,scheme", [("A", 2), ("E", 3)])
+def test_calc_fp32_refined_matches_fp64(emtd, tmp_path, form, scheme):
+    """FP32 factor + FP64 refinement: receivers within 1e-5 of the FP64 path and the oracle."""
+    d, _ = make_model(tmp_path, cfg=3, cells=(10, 10, 8), steps=(5, 3), form=form, scheme=scheme)
+    print("case fp32-ir", form, scheme, flush=True)
+    fe = 0 if form == "A" else 1
+    c64 = emtd.Calc(d, fe, scheme, emtd.FP64)
+    c64.run()
+    c32 = emtd.Calc(d, fe, scheme, emtd.FP32_IR)
+    c32.run()
+    inf = c32.info()
+    print("refine iterations", inf.refine_iterations, "fallbacks", inf.fallbacks, "last residual",
+          inf.last_residual)
+    for kind in (0, 1):
+        a = c64.receiver_values(kind)
+        b = c32.receiver_values(kind)
+        err = np.abs(a - b).max() / np.abs(a).max()
+        print("kind", kind, "fp32-ir vs fp64 max rel err", err)
+        assert err < TOL32
+    assert inf.last_residual <= 1e-10 or inf.fallbacks > 0
