@@ -273,6 +273,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Stages the nb x nb diagonal block L[c0.., c0..] of a front into LDS (D[row][col]) with
+// coalesced column reads, so the sequential triangular solve reads LDS instead of HBM.
+template <typename T>
+__device__ __forceinline__ void stage_diag(double (*D)[SB + 1], const T* Lf, int m, int c0, int nb) {
+    for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x) {
+        const int i = idx & (SB - 1), j = idx / SB;
+        D[i][j] = (i < nb && j < nb && i >= j) ? double(Lf[int64_t(c0 + j) * m + c0 + i]) : 0.0;
+    }
+}
+
 // Forward sweep of one level: one workgroup per front and right-hand side.
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd(const DevFront* __restrict__ F, const int* __restrict__ flist,
@@ -296,16 +306,19 @@ __global__ void __launch_bounds__(256) k_fwd(const DevFront* __restrict__ F, con
         __syncthreads();
     }
     __shared__ double y[SB];
+    __shared__ double D[SB][SB + 1];
     const T* Lf = L + Fr.loff;
     const int lane = threadIdx.x & 63;
     for (int kb = 0; kb < p; kb += SB) {
         const int nb = min(SB, p - kb);
+        stage_diag(D, Lf, m, kb, nb);
+        __syncthreads();
         if (threadIdx.x < 64) {
             double val = lane < nb ? v[kb + lane] : 0.0;
             for (int j = 0; j < nb; ++j) {
-                const double yj = __shfl(val, j, 64) / double(Lf[int64_t(kb + j) * m + kb + j]);
+                const double yj = __shfl(val, j, 64) / D[j][j];
                 if (lane == j) y[j] = yj;
-                if (lane > j && lane < nb) val -= double(Lf[int64_t(kb + j) * m + kb + lane]) * yj;
+                if (lane > j && lane < nb) val -= D[lane][j] * yj;
             }
             if (lane < nb) v[kb + lane] = y[lane];
         }
@@ -336,6 +349,7 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
     for (int i = threadIdx.x; i < m; i += blockDim.x) v[i] = i < p ? xx[Fr.fbeg + i] : xx[rw[i]];
     __syncthreads();
     __shared__ double s[SB];
+    __shared__ double D[SB][SB + 1];
     const T* Lf = L + Fr.loff;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwaves = blockDim.x >> 6;
@@ -348,13 +362,14 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
             part = wave_sum(part);
             if (lane == 0) s[j] = part;
         }
+        stage_diag(D, Lf, m, kb, nb);
         __syncthreads();
         if (threadIdx.x < 64) {
             double val = lane < nb ? v[kb + lane] - s[lane] : 0.0;
             for (int j = nb - 1; j >= 0; --j) {
-                const double xj = __shfl(val, j, 64) / double(Lf[int64_t(kb + j) * m + kb + j]);
+                const double xj = __shfl(val, j, 64) / D[j][j];
                 if (lane == j) s[j] = xj;
-                if (lane < j) val -= double(Lf[int64_t(kb + lane) * m + kb + j]) * xj;
+                if (lane < j) val -= D[j][lane] * xj;
             }
             if (lane < nb) v[kb + lane] = s[lane];
         }
@@ -416,155 +431,234 @@ __global__ void __launch_bounds__(256) k_bwd_gather(const DevFront* __restrict__
     for (int i = threadIdx.x; i < Fr.m; i += blockDim.x) v[i] = i < Fr.p ? x[Fr.fbeg + i] : x[rw[i]];
 }
 
-// item.x = front, item.y = 64-row block; flag_off per front in foff[]
+// ------------------------------------------------------------------ split sweeps
+// Levels with fewer fronts use three kernels per sweep so that only the pivot block L11 of a
+// front goes through a dependency chain and the rest of the factor streams at full width:
+//   forward:  chain over the 64-row pivot blocks of L11 (y_b = inv(L_bb) (v_b - sum L_bc y_c)),
+//             then w = v_U - L21 y as a wait-free GEMV over 64-row blocks of U rows;
+//   backward: t = L21^T x_U as wait-free partial sums over 1024-row chunks, then the chain
+//             x_b = inv(L_bb)^T (y_b - t_b - sum_{c>b} L_cb^T x_c) from the last block down.
+// inv(L_bb) (64x64, row-major, zero-padded) is computed once per factorization.
+constexpr int RCH = 1024;  // rows per partial sum of the backward L21^T product
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_big(const DevFront* __restrict__ F, const int2* __restrict__ items,
-                                                 const int* __restrict__ foff, int* __restrict__ flags,
-                                                 double* __restrict__ ybuf, int* __restrict__ counter,
-                                                 int* __restrict__ err, const T* __restrict__ L,
-                                                 double* __restrict__ W, double* __restrict__ x) {
+__global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                               const int64_t* __restrict__ ioff, const T* __restrict__ L,
+                                               T* __restrict__ inv) {
+    __shared__ double A[SB][SB + 1];
+    __shared__ double X[SB][SB + 1];
+    const int2 it = items[blockIdx.x];
+    const DevFront Fr = F[it.x];
+    const int c0 = 64 * it.y, nb = min(64, Fr.p - c0);
+    const T* Lf = L + Fr.loff;
+    const int t = threadIdx.x;
+    for (int j = 0; j < SB; ++j) A[t][j] = (t < nb && j < nb && t >= j) ? double(Lf[int64_t(c0 + j) * Fr.m + c0 + t]) : 0.0;
+    __syncthreads();
+    // column t of the inverse by forward substitution
+    for (int i = 0; i < SB; ++i) {
+        double v = 0.0;
+        if (t < nb && i < nb && i >= t) {
+            v = (i == t) ? 1.0 : 0.0;
+            for (int k = t; k < i; ++k) v -= A[i][k] * X[k][t];
+            v /= A[i][i];
+        }
+        X[i][t] = v;
+    }
+    T* out = inv + ioff[it.x] + int64_t(it.y) * SB * SB;
+    for (int i = 0; i < SB; ++i) out[i * SB + t] = T(X[i][t]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                   const int* __restrict__ foff, int* __restrict__ flags,
+                                                   double* __restrict__ ybuf, int* __restrict__ counter,
+                                                   int* __restrict__ err, const T* __restrict__ L,
+                                                   const T* __restrict__ inv, const int64_t* __restrict__ ioff,
+                                                   double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
     __shared__ double red[4][64];
-    __shared__ double ys[64];
+    __shared__ double rs[64];
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
     const int2 it = items[s_ticket];
     const DevFront Fr = F[it.x];
-    const int rb = it.y;
-    const int p = Fr.p, m = Fr.m;
-    const int np = (p + 63) / 64;
-    const int r0 = 64 * rb;
+    const int rb = it.y, p = Fr.p, m = Fr.m;
+    const int r0 = 64 * rb, nbp = min(64, p - r0);
     const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int r = r0 + lr;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     int* fl = flags + foff[it.x];
     double* yb = ybuf + int64_t(foff[it.x]) * 64;
-    const int ncb = (r0 < p) ? rb : np;
+    // this thread's 16 entries of inv(L_bb) (row lr, columns 16q..16q+15), loaded before any wait
+    const T* iv = inv + ioff[it.x] + int64_t(rb) * SB * SB + lr * SB + 16 * q;
+    double ivr[16];
+    for (int j = 0; j < 16; ++j) ivr[j] = double(iv[j]);
     double acc = 0.0;
-    for (int cb = 0; cb < ncb; ++cb) {
+    for (int cb = 0; cb < rb; ++cb) {
+        double lv[16];
+        const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
+        for (int j = 0; j < 16; ++j) lv[j] = lr < nbp ? double(lc[int64_t(j) * m]) : 0.0;
         if (threadIdx.x == 0) wait_flag(fl + cb, err);
         __syncthreads();
-        const int nbc = min(64, p - 64 * cb);
-        double yv[16];
-        for (int j = 0; j < 16; ++j) {
-            const int jj = 16 * q + j;
-            yv[j] = jj < nbc ? __hip_atomic_load(yb + 64 * cb + jj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        }
+        for (int j = 0; j < 16; ++j)
+            acc += lv[j] * __hip_atomic_load(yb + 64 * cb + 16 * q + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    red[q][lr] = acc;
+    __syncthreads();
+    if (q == 0) rs[lr] = lr < nbp ? v[r] - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]) : 0.0;
+    __syncthreads();
+    // y = inv(L_bb) rs : thread (row lr, quarter q) sums 16 columns
+    double part = 0.0;
+    for (int j = 0; j < 16; ++j) part += ivr[j] * rs[16 * q + j];
+    red[q][lr] = part;
+    __syncthreads();
+    if (q == 0 && lr < nbp) {
+        const double y = red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr];
+        __hip_atomic_store(yb + 64 * rb + lr, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v[r] = y;
+        x[Fr.fbeg + r] = y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) publish_flag(fl + rb);
+}
+
+// w = v_U - L21 y for one 64-row block of update rows (y final: previous kernel).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_l21(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                 const T* __restrict__ L, double* __restrict__ W) {
+    __shared__ double ys[1024];
+    __shared__ double red[4][64];
+    const int2 it = items[blockIdx.x];
+    const DevFront Fr = F[it.x];
+    const int p = Fr.p, m = Fr.m;
+    const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = p + 64 * it.y + lr;
+    const T* Lf = L + Fr.loff;
+    double* v = W + Fr.woff;
+    double acc = 0.0;
+    for (int c0 = 0; c0 < p; c0 += 1024) {
+        const int nc = min(1024, p - c0);
+        __syncthreads();
+        for (int j = threadIdx.x; j < nc; j += blockDim.x) ys[j] = v[c0 + j];
+        __syncthreads();
         if (r < m) {
-            const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
-            for (int j = 0; j < 16; ++j)
-                if (16 * q + j < nbc) acc += double(lc[int64_t(j) * m]) * yv[j];
+            const T* lc = Lf + int64_t(c0) * m + r;
+            int j = q;
+            for (; j + 12 < nc; j += 16) {
+                const double a0 = double(lc[int64_t(j) * m]), a1 = double(lc[int64_t(j + 4) * m]);
+                const double a2 = double(lc[int64_t(j + 8) * m]), a3 = double(lc[int64_t(j + 12) * m]);
+                acc += a0 * ys[j] + a1 * ys[j + 4] + a2 * ys[j + 8] + a3 * ys[j + 12];
+            }
+            for (; j < nc; j += 4) acc += double(lc[int64_t(j) * m]) * ys[j];
         }
     }
     red[q][lr] = acc;
     __syncthreads();
-    double rowv = 0.0;
-    if (q == 0 && r < m) rowv = v[r] - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]);
-    if (r0 < p) {
-        const int nbp = min(64, p - r0);
-        if (q == 0) {
-            double val = rowv;
-            for (int j = 0; j < nbp; ++j) {
-                const double yj = __shfl(val, j, 64) / double(Lf[int64_t(r0 + j) * m + r0 + j]);
-                if (lr == j) ys[j] = yj;
-                if (lr > j && lr < nbp) val -= double(Lf[int64_t(r0 + j) * m + r0 + lr]) * yj;
-            }
-            if (lr < nbp) {
-                const double y = ys[lr];
-                __hip_atomic_store(yb + 64 * rb + lr, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                v[r] = y;
-                x[Fr.fbeg + r] = y;
-            }
+    if (q == 0 && r < m) v[r] -= red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr];
+}
+
+// part[chunk][c] = sum over the chunk's update rows of L[r][c] x[r], for 64 pivot columns.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F, const int3* __restrict__ items,
+                                                  const int64_t* __restrict__ poff, const T* __restrict__ L,
+                                                  const double* __restrict__ W, double* __restrict__ part) {
+    const int3 it = items[blockIdx.x];
+    const DevFront Fr = F[it.x];
+    const int p = Fr.p, m = Fr.m;
+    const int c0 = 64 * it.y, nbc = min(64, p - c0);
+    const int rb = p + RCH * it.z, re = min(m, rb + RCH);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const T* Lf = L + Fr.loff;
+    const double* v = W + Fr.woff;
+    double acc[16];
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    for (int r = rb + lane; r < re; r += 64) {
+        const double xv = v[r];
+        for (int j = 0; j < 16; ++j) {
+            const int c = wave + 4 * j;
+            if (c < nbc) acc[j] += double(Lf[int64_t(c0 + c) * m + r]) * xv;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) publish_flag(fl + rb);
-        // update rows sharing this block: subtract the block's own contribution
-        if (q == 0 && lr >= nbp && r < m) {
-            double a2 = rowv;
-            for (int j = 0; j < nbp; ++j) a2 -= double(Lf[int64_t(r0 + j) * m + r]) * ys[j];
-            v[r] = a2;
-        }
-    } else if (q == 0 && r < m) {
-        v[r] = rowv;
+    }
+    double* out = part + poff[it.x] + int64_t(it.z) * p + c0;
+    for (int j = 0; j < 16; ++j) {
+        double s = acc[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        const int c = wave + 4 * j;
+        if (lane == 0 && c < nbc) out[c] = s;
     }
 }
 
-// item.x = front, item.y = 64-column pivot block (items in decreasing block order per front)
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_big(const DevFront* __restrict__ F, const int2* __restrict__ items,
-                                                 const int* __restrict__ foff, int* __restrict__ flags,
-                                                 double* __restrict__ xbuf, int* __restrict__ counter,
-                                                 int* __restrict__ err, const T* __restrict__ L,
-                                                 double* __restrict__ W, double* __restrict__ x) {
+__global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                   const int* __restrict__ foff, int* __restrict__ flags,
+                                                   double* __restrict__ xbuf, int* __restrict__ counter,
+                                                   int* __restrict__ err, const T* __restrict__ L,
+                                                   const T* __restrict__ inv, const int64_t* __restrict__ ioff,
+                                                   const double* __restrict__ part, const int64_t* __restrict__ poff,
+                                                   double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
     __shared__ double sred[64];
-    __shared__ double xs[64];
-    __shared__ double xr[64];
+    __shared__ double rs[64];
+    __shared__ double red[4][64];
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
     const int2 it = items[s_ticket];
     const DevFront Fr = F[it.x];
-    const int cb = it.y;
-    const int p = Fr.p, m = Fr.m;
-    const int np = (p + 63) / 64;
-    const int c0 = 64 * cb;
-    const int nbc = min(64, p - c0);
+    const int cb = it.y, p = Fr.p, m = Fr.m;
+    const int np = (p + 63) / 64, nchunk = (m - p + RCH - 1) / RCH;
+    const int c0 = 64 * cb, nbc = min(64, p - c0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     int* fl = flags + foff[it.x];
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
-    double part[16];
-    for (int j = 0; j < 16; ++j) part[j] = 0.0;
-    // update rows first (their values are final), in 64-row chunks
-    for (int rr = p; rr < m; rr += 64) {
-        const int r = rr + lane;
-        const double xv = r < m ? v[r] : 0.0;
-        if (r < m)
-            for (int j = 0; j < 16; ++j) {
-                const int jj = wave + 4 * j;
-                if (jj < nbc) part[j] += double(Lf[int64_t(c0 + jj) * m + r]) * xv;
-            }
-    }
-    // later pivot blocks, newest first
+    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15), loaded before any wait
+    const T* iv = inv + ioff[it.x] + int64_t(cb) * SB * SB + lane;
+    double ivc[16];
+    for (int j = 0; j < 16; ++j) ivc[j] = double(iv[(16 * wave + j) * SB]);
+    double acc[16];
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
     for (int cb2 = np - 1; cb2 > cb; --cb2) {
+        const int nb2 = min(64, p - 64 * cb2);
+        double lv[16];
+        for (int j = 0; j < 16; ++j) {
+            const int c = wave + 4 * j;
+            lv[j] = (lane < nb2 && c < nbc) ? double(Lf[int64_t(c0 + c) * m + 64 * cb2 + lane]) : 0.0;
+        }
         if (threadIdx.x == 0) wait_flag(fl + cb2, err);
         __syncthreads();
-        const int nb2 = min(64, p - 64 * cb2);
-        if (threadIdx.x < 64)
-            xr[lane] = lane < nb2 ? __hip_atomic_load(xb + 64 * cb2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                  : 0.0;
-        __syncthreads();
-        const int r = 64 * cb2 + lane;
-        if (lane < nb2)
-            for (int j = 0; j < 16; ++j) {
-                const int jj = wave + 4 * j;
-                if (jj < nbc) part[j] += double(Lf[int64_t(c0 + jj) * m + r]) * xr[lane];
-            }
-        __syncthreads();
+        const double xv =
+            lane < nb2 ? __hip_atomic_load(xb + 64 * cb2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        for (int j = 0; j < 16; ++j) acc[j] += lv[j] * xv;
     }
-    // rows below the diagonal inside the block itself are handled by the triangular solve
     for (int j = 0; j < 16; ++j) {
-        double s = part[j];
+        double s = acc[j];
         for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        const int jj = wave + 4 * j;
-        if (lane == 0 && jj < 64) sred[jj] = s;
+        if (lane == 0) sred[wave + 4 * j] = s;
     }
     __syncthreads();
     if (threadIdx.x < 64) {
-        double val = lane < nbc ? v[c0 + lane] - sred[lane] : 0.0;
-        for (int j = nbc - 1; j >= 0; --j) {
-            const double xj = __shfl(val, j, 64) / double(Lf[int64_t(c0 + j) * m + c0 + j]);
-            if (lane == j) xs[j] = xj;
-            if (lane < j) val -= double(Lf[int64_t(c0 + lane) * m + c0 + j]) * xj;
-        }
+        double t = 0.0;
         if (lane < nbc) {
-            const double xv = xs[lane];
-            __hip_atomic_store(xb + 64 * cb + lane, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v[c0 + lane] = xv;
-            x[Fr.fbeg + c0 + lane] = xv;
+            const double* pp = part + poff[it.x] + c0 + lane;
+            for (int k = 0; k < nchunk; ++k) t += pp[int64_t(k) * p];
+            rs[lane] = v[c0 + lane] - t - sred[lane];
+        } else {
+            rs[lane] = 0.0;
         }
+    }
+    __syncthreads();
+    // x_b = inv(L_bb)^T rs : thread (column lane, quarter) sums 16 rows of inv
+    double pt = 0.0;
+    for (int j = 0; j < 16; ++j) pt += ivc[j] * rs[16 * wave + j];
+    red[wave][lane] = pt;
+    __syncthreads();
+    if (threadIdx.x < 64 && lane < nbc) {
+        const double xv = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+        __hip_atomic_store(xb + 64 * cb + lane, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v[c0 + lane] = xv;
+        x[Fr.fbeg + c0 + lane] = xv;
     }
     __syncthreads();
     if (threadIdx.x == 0) publish_flag(fl + cb);
@@ -700,6 +794,7 @@ void GpuCholesky<T>::factorize_system(const int64_t* d_dest_off, const int64_t* 
     hipLaunchKernelGGL(k_scatter_system_diag<T>, dim3((n + 255) / 256), dim3(256), 0, stream_, d_dest_diag, d_b_di,
                        d_c_di, n, mult, L_.p);
     numeric();
+    invert_diagonal_blocks();
     EMTD_HIP(hipEventRecord(e1, stream_));
     EMTD_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -727,6 +822,7 @@ void GpuCholesky<T>::factorize_values(const int64_t* d_dest_off, const int64_t* 
     hipLaunchKernelGGL(k_scatter_values<T>, dim3((n + 255) / 256), dim3(256), 0, stream_, d_dest_diag, d_di,
                        int64_t(n), L_.p);
     numeric();
+    invert_diagonal_blocks();
     EMTD_HIP(hipEventRecord(e1, stream_));
     EMTD_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -748,43 +844,101 @@ template <typename T>
 void GpuCholesky<T>::prepare_big_levels() {
     const Symbolic& S = S_;
     big_.assign(S.nlevels, 0);
-    std::vector<int2> fi, bi;
+    std::vector<int2> fc, l21, bc;
+    std::vector<int3> l21t;
     std::vector<int> foff(S.nf, 0);
-    big_fwd_ptr_.assign(S.nlevels + 1, 0);
-    big_bwd_ptr_.assign(S.nlevels + 1, 0);
-    int64_t nflags = 0;
-    int maxflags = 0;
+    std::vector<int64_t> poff(S.nf, 0);
+    lv_fc_.assign(S.nlevels + 1, 0);
+    lv_l21_.assign(S.nlevels + 1, 0);
+    lv_l21t_.assign(S.nlevels + 1, 0);
+    lv_bc_.assign(S.nlevels + 1, 0);
+    int maxflags = 1;
+    int64_t maxpart = 1;
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
-        int maxp = 0;
-        for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) maxp = std::max(maxp, S.pcount[S.level_fronts[k]]);
-        big_[l] = (nfl < big_level_fronts_ && maxp > 64) ? 1 : 0;
+        big_[l] = nfl < big_level_fronts_ ? 1 : 0;
         if (big_[l]) {
-            int off = 0;
+            int off = 0, maxnp = 0;
+            int64_t po = 0;
             for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
                 const int f = S.level_fronts[k];
+                const int p = S.pcount[f], u = S.mcount[f] - p;
+                const int np = (p + 63) / 64, nch = (u + RCH - 1) / RCH;
                 foff[f] = off;
-                const int np = (S.pcount[f] + 63) / 64, nr = (S.mcount[f] + 63) / 64;
-                for (int rb = 0; rb < nr; ++rb) fi.push_back(make_int2(f, rb));
-                for (int cb = np - 1; cb >= 0; --cb) bi.push_back(make_int2(f, cb));
+                poff[f] = po;
                 off += np;
+                po += int64_t(nch) * p;
+                maxnp = std::max(maxnp, np);
+                for (int rb = 0; rb < (u + 63) / 64; ++rb) l21.push_back(make_int2(f, rb));
+                for (int cb = 0; cb < np; ++cb)
+                    for (int ch = 0; ch < nch; ++ch) l21t.push_back(make_int3(f, cb, ch));
             }
+            // chains interleaved across the level's fronts, increasing (forward) / decreasing (backward)
+            for (int t = 0; t < maxnp; ++t)
+                for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
+                    const int f = S.level_fronts[k];
+                    const int np = (S.pcount[f] + 63) / 64;
+                    if (t < np) {
+                        fc.push_back(make_int2(f, t));
+                        bc.push_back(make_int2(f, np - 1 - t));
+                    }
+                }
             maxflags = std::max(maxflags, off);
+            maxpart = std::max(maxpart, po);
         }
-        big_fwd_ptr_[l + 1] = int(fi.size());
-        big_bwd_ptr_[l + 1] = int(bi.size());
+        lv_fc_[l + 1] = int(fc.size());
+        lv_l21_[l + 1] = int(l21.size());
+        lv_l21t_[l + 1] = int(l21t.size());
+        lv_bc_[l + 1] = int(bc.size());
     }
-    (void)nflags;
-    big_fwd_items_ = to_device(fi.empty() ? std::vector<int2>{make_int2(0, 0)} : fi, stream_);
-    big_bwd_items_ = to_device(bi.empty() ? std::vector<int2>{make_int2(0, 0)} : bi, stream_);
+    auto nz2 = [](std::vector<int2> v) { if (v.empty()) v.push_back(make_int2(0, 0)); return v; };
+    fc_items_ = to_device(nz2(fc), stream_);
+    l21_items_ = to_device(nz2(l21), stream_);
+    bc_items_ = to_device(nz2(bc), stream_);
+    if (l21t.empty()) l21t.push_back(make_int3(0, 0, 0));
+    l21t_items_ = to_device(l21t, stream_);
     foff_ = to_device(foff, stream_);
-    max_flags_ = std::max(maxflags, 1);
-    flags_.alloc(size_t(max_flags_) + 2);   // flags + ticket counter + error word
+    poff_ = to_device(poff, stream_);
+    max_flags_ = maxflags;
+    flags_.alloc(size_t(max_flags_) + 2);  // flags + ticket counter + error word
     xbuf_.alloc(size_t(max_flags_) * 64);
+    part_.alloc(size_t(maxpart));
     std::vector<int> z(max_flags_ + 2, 0);
     flags_.upload(z.data(), z.size(), stream_);
     EMTD_HIP(hipStreamSynchronize(stream_));
     big_ready_ = true;
+}
+
+template <typename T>
+void GpuCholesky<T>::prepare_inverses() {
+    const Symbolic& S = S_;
+    if (!big_ready_) prepare_big_levels();
+    // only fronts of split-sweep levels use the inverted pivot blocks
+    std::vector<int2> items;
+    std::vector<int64_t> ioff(S.nf, 0);
+    int64_t o = 0;
+    for (int l = 0; l < S.nlevels; ++l) {
+        if (!big_[l]) continue;
+        for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
+            const int f = S.level_fronts[k];
+            const int np = (S.pcount[f] + 63) / 64;
+            ioff[f] = o;
+            o += int64_t(np) * SB * SB;
+            for (int b = 0; b < np; ++b) items.push_back(make_int2(f, b));
+        }
+    }
+    inv_items_ = to_device(items, stream_);
+    ioff_ = to_device(ioff, stream_);
+    inv_.alloc(size_t(std::max<int64_t>(o, 1)));
+    n_inv_items_ = int(items.size());
+}
+
+template <typename T>
+void GpuCholesky<T>::invert_diagonal_blocks() {
+    if (!inv_.p) prepare_inverses();
+    if (n_inv_items_ > 0)
+        hipLaunchKernelGGL(k_invdiag<T>, dim3(n_inv_items_), dim3(64), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
+                           L_.p, inv_.p);
 }
 
 template <typename T>
@@ -806,6 +960,10 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
     }
     int* counter = flags_.p + max_flags_;
     int* err = flags_.p + max_flags_ + 1;
+    auto clear_flags = [&]() {
+        hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
+                           int64_t(max_flags_ + 1), 0);
+    };
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         const int* lf = level_fronts_.p + S.level_ptr[l];
@@ -814,14 +972,17 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                relmap_.p, L_.p, W_.p, S.wsize, d_b, d_x, ld);
             continue;
         }
-        const int b = big_fwd_ptr_[l], e = big_fwd_ptr_[l + 1];
         for (int r = 0; r < nrhs; ++r) {
-            hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
-                               int64_t(max_flags_ + 1), 0);
+            double* Wr = W_.p + r * S.wsize;
+            clear_flags();
             hipLaunchKernelGGL(k_fwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, children_.p, relmap_.p,
-                               W_.p + r * S.wsize, d_b + r * ld);
-            hipLaunchKernelGGL(k_fwd_big<T>, dim3(e - b), dim3(256), 0, stream_, fronts_.p, big_fwd_items_.p + b,
-                               foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, W_.p + r * S.wsize, d_x + r * ld);
+                               Wr, d_b + r * ld);
+            hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
+                               fc_items_.p + lv_fc_[l], foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               ioff_.p, Wr, d_x + r * ld);
+            if (lv_l21_[l + 1] > lv_l21_[l])
+                hipLaunchKernelGGL(k_fwd_l21<T>, dim3(lv_l21_[l + 1] - lv_l21_[l]), dim3(256), 0, stream_, fronts_.p,
+                                   l21_items_.p + lv_l21_[l], L_.p, Wr);
         }
     }
     for (int l = S.nlevels - 1; l >= 0; --l) {
@@ -832,14 +993,17 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                S.wsize, d_x, ld);
             continue;
         }
-        const int b = big_bwd_ptr_[l], e = big_bwd_ptr_[l + 1];
         for (int r = 0; r < nrhs; ++r) {
-            hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
-                               int64_t(max_flags_ + 1), 0);
-            hipLaunchKernelGGL(k_bwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, rows_.p,
-                               W_.p + r * S.wsize, d_x + r * ld);
-            hipLaunchKernelGGL(k_bwd_big<T>, dim3(e - b), dim3(256), 0, stream_, fronts_.p, big_bwd_items_.p + b,
-                               foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, W_.p + r * S.wsize, d_x + r * ld);
+            double* Wr = W_.p + r * S.wsize;
+            hipLaunchKernelGGL(k_bwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, rows_.p, Wr,
+                               d_x + r * ld);
+            if (lv_l21t_[l + 1] > lv_l21t_[l])
+                hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(lv_l21t_[l + 1] - lv_l21t_[l]), dim3(256), 0, stream_,
+                                   fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
+            clear_flags();
+            hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
+                               bc_items_.p + lv_bc_[l], foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);
         }
     }
     EMTD_HIP(hipGetLastError());

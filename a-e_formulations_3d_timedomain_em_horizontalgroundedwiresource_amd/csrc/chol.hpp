@@ -44,7 +44,6 @@ class GpuCholesky {
     // Nonzero when a large-front sweep hit its wait limit (synchronizes).
     int sweep_error();
     // Levels with fewer fronts than this use the multi-workgroup sweeps.
-    void set_big_level_fronts(int k) { big_level_fronts_ = k; big_ready_ = false; }
     double last_factor_seconds() const { return factor_seconds_; }
 
     const Symbolic& sym() const { return S_; }
@@ -53,6 +52,8 @@ class GpuCholesky {
   private:
     void numeric();
     void prepare_big_levels();
+    void prepare_inverses();
+    void invert_diagonal_blocks();
     const Symbolic& S_;
     hipStream_t stream_;
     DevBuf<T> L_, U_;
@@ -66,13 +67,17 @@ class GpuCholesky {
     std::vector<int> level_sorted_off_;
     DevBuf<int> info_;
     int wrhs_ = 0;
-    int big_level_fronts_ = 64;
+    int big_level_fronts_ = 1024;
     bool big_ready_ = false;
     std::vector<char> big_;
-    std::vector<int> big_fwd_ptr_, big_bwd_ptr_;
-    DevBuf<int2> big_fwd_items_, big_bwd_items_;
+    std::vector<int> lv_fc_, lv_l21_, lv_l21t_, lv_bc_;
+    DevBuf<int2> fc_items_, l21_items_, bc_items_, inv_items_;
+    DevBuf<int3> l21t_items_;
     DevBuf<int> foff_, flags_;
-    DevBuf<double> xbuf_;
+    DevBuf<int64_t> poff_, ioff_;
+    DevBuf<double> xbuf_, part_;
+    DevBuf<T> inv_;
+    int n_inv_items_ = 0;
     int max_flags_ = 1;
     double factor_seconds_ = 0;
 };
