@@ -6,9 +6,11 @@
 //
 //   scatter    system entries  -> L storage            (one thread per matrix entry)
 //   extend-add child update U_c -> parent front          (one wavefront per update column)
-//   potrf      diagonal NB x NB block of each front      (one workgroup per front, LDS)
-//   trsm       panel rows below the diagonal block       (one wavefront per 64 rows)
-//   update     trailing lower matrix -= panel * panel^T  (64x64 tiles, MFMA 16x16x4)
+//   panel      diagonal NB x NB block (factored in LDS by every workgroup) and the panel
+//              rows below it                             (one wavefront per 64 rows)
+//   syrk       update inside the 128-column outer block  (64x64 tiles, MFMA 16x16x4)
+//   update     trailing pivot columns per outer block, and the update block U once per
+//              front with K = p                          (128x128 tiles, MFMA 16x16x4)
 //
 // Storage per front f with p pivots, u update rows, m = p + u: L_f is m x p column-major
 // (ld = m) and holds the factor columns; U_f is u x u column-major and receives the Schur
@@ -107,74 +109,66 @@ __global__ void k_extend_add(const DevFront* __restrict__ F, const int2* __restr
     }
 }
 
-// Unblocked Cholesky of the diagonal block [kb, kb+nb) of each active front.
+// One panel step of the blocked factorization, for the columns [kb, kb+nb) of each active
+// front: every workgroup factors the nb x nb diagonal block in LDS (the same arithmetic in
+// every workgroup, so all copies agree), then solves X * L_kk^T = B for its 64 rows below the
+// diagonal block. The diagonal block may only be overwritten once every workgroup of the front
+// has read it: each workgroup counts itself in done[front] after its reads, and the last one
+// stores the factored block, reports the first non-positive pivot (as a global index) through
+// atomicMin(info) and resets the counter for the next panel.
 template <typename T>
-__global__ void k_potrf(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
-                        T* __restrict__ L, int* __restrict__ info) {
-    const DevFront Fr = F[flist[blockIdx.x]];
-    if (kb >= Fr.p) return;
-    const int nb = min(NB, Fr.p - kb);
-    __shared__ T A[NB][NB + 1];
-    __shared__ int bad;
-    T* base = L + Fr.loff + int64_t(kb) * Fr.m + kb;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb, j = idx / nb;
-        if (i >= j) A[i][j] = base[int64_t(j) * Fr.m + i];
-    }
-    if (threadIdx.x == 0) bad = -1;
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (threadIdx.x == 0) {
-            T d = A[j][j];
-            if (!(d > T(0))) {
-                if (bad < 0) bad = j;
-                d = T(1);
-            }
-            A[j][j] = sqrt(d);
-        }
-        __syncthreads();
-        const T dj = A[j][j];
-        for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) A[i][j] /= dj;
-        __syncthreads();
-        const int w = nb - j - 1;
-        for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
-            const int i = j + 1 + idx % w, l = j + 1 + idx / w;
-            if (l <= i) A[i][l] -= A[i][j] * A[l][j];
-        }
-        __syncthreads();
-    }
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb, j = idx / nb;
-        if (i >= j) base[int64_t(j) * Fr.m + i] = A[i][j];
-    }
-    if (threadIdx.x == 0 && bad >= 0) atomicMin(info, Fr.fbeg + kb + bad);
-}
-
-// Panel rows below the diagonal block: X * L_kk^T = B, 64 rows per wavefront.
-template <typename T>
-__global__ void k_trsm(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb, T* __restrict__ L) {
-    const DevFront Fr = F[flist[blockIdx.y]];
+__global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
+                                              T* __restrict__ L, int* __restrict__ info, int* __restrict__ done) {
+    const int f = flist[blockIdx.y];
+    const DevFront Fr = F[f];
     if (kb >= Fr.p) return;
     const int nb = min(NB, Fr.p - kb);
     const int r0 = kb + nb + 64 * blockIdx.x;
-    if (r0 >= Fr.m) return;
-    __shared__ T Lkk[NB][NB + 1];
+    if (blockIdx.x > 0 && r0 >= Fr.m) return;
+    const int nwg = max(1, (Fr.m - kb - nb + 63) / 64);
+    __shared__ int s_last;
+    __shared__ T A[NB][NB + 1];
     __shared__ T X[64][NB + 1];
     T* col = L + Fr.loff + int64_t(kb) * Fr.m;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb, j = idx / nb;
-        if (i >= j) Lkk[i][j] = col[int64_t(j) * Fr.m + kb + i];
-    }
     const int t = threadIdx.x;
+    for (int j = 0; j < nb; ++j)
+        if (t < nb) A[t][j] = t >= j ? col[int64_t(j) * Fr.m + kb + t] : T(0);
     const int r = r0 + t;
     if (r < Fr.m)
         for (int j = 0; j < nb; ++j) X[t][j] = col[int64_t(j) * Fr.m + r];
     __syncthreads();
+    if (t == 0) s_last = atomicAdd(done + f, 1) == nwg - 1;
+    int bad = -1;
+    const int l = t & 31, half = t >> 5;
+    for (int j = 0; j < nb; ++j) {
+        T d = A[j][j];
+        if (!(d > T(0))) {
+            if (bad < 0) bad = j;
+            d = T(1);
+        }
+        d = sqrt(d);
+        __syncthreads();
+        if (t == 0) A[j][j] = d;
+        if (t > j && t < nb) A[t][j] /= d;
+        __syncthreads();
+        if (l > j && l < nb)
+            for (int i = j + 1 + half; i < nb; i += 2)
+                if (l <= i) A[i][l] -= A[i][j] * A[l][j];
+        __syncthreads();
+    }
+    if (s_last) {
+        for (int j = 0; j < nb; ++j)
+            if (t < nb && t >= j) col[int64_t(j) * Fr.m + kb + t] = A[t][j];
+        if (t == 0) {
+            if (bad >= 0) atomicMin(info, Fr.fbeg + kb + bad);
+            done[f] = 0;
+        }
+    }
     if (r < Fr.m) {
         for (int j = 0; j < nb; ++j) {
-            T s = X[t][j];
-            for (int l = 0; l < j; ++l) s -= X[t][l] * Lkk[j][l];
-            X[t][j] = s / Lkk[j][j];
+            T v = X[t][j];
+            for (int k = 0; k < j; ++k) v -= X[t][k] * A[j][k];
+            X[t][j] = v / A[j][j];
         }
         for (int j = 0; j < nb; ++j) col[int64_t(j) * Fr.m + r] = X[t][j];
     }
@@ -202,23 +196,21 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
-// Symmetric rank-k update of the lower frontal matrix with the factored columns
-// [k0, k0+kd), kd = min(kmax, p-k0):  F[r][c] -= sum_k L[r][k] L[c][k]  for r >= c in the
-// target columns [k0+kd, ce), where ce = min(kb+NBO, p) (mode 0: update inside the current
-// 128-column block) or m (mode 1: the trailing matrix). 64x64 tiles (blockIdx.x = row tile,
-// blockIdx.y = column tile, blockIdx.z = front); 4 wavefronts own a 32x32 quarter each as
-// 2x2 MFMA 16x16x4 blocks (lane layouts in Mfma<T>);
-// K is streamed through LDS in chunks of 32 and the tile is read and written once.
+// Symmetric rank-k update inside the current outer block: with the factored columns
+// [k0, k0+kd), kd = min(kmax, p-k0),  F[r][c] -= sum_k L[r][k] L[c][k]  for r >= c in the
+// target columns [k0+kd, min(kb+NBO, p)). 64x64 tiles (blockIdx.x = row tile, blockIdx.y =
+// column tile, blockIdx.z = front); 4 wavefronts own a 32x32 quarter each as 2x2 MFMA 16x16x4
+// blocks (lane layouts in Mfma<T>); K is streamed through LDS in chunks of 32.
 constexpr int NBO = 128;  // outer block of the factorization
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_syrk(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
-                                              int k0, int kmax, int mode, T* __restrict__ L, T* __restrict__ U) {
+                                              int k0, int kmax, T* __restrict__ L, T* __restrict__ U) {
     const DevFront Fr = F[flist[blockIdx.z]];
     const int kd = min(kmax, Fr.p - k0);
     if (kd <= 0) return;
     const int cs = k0 + kd;
-    const int ce = mode == 0 ? min(kb + NBO, Fr.p) : Fr.m;
+    const int ce = min(kb + NBO, Fr.p);
     if (cs >= ce) return;
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (ti < tj) return;
@@ -265,6 +257,111 @@ __global__ void __launch_bounds__(256) k_syrk(const DevFront* __restrict__ F, co
                 const int c = C0 + wc + 16 * b + li;
                 if (r < Fr.m && c < ce && r >= c) *front_elem(Fr, L, U, r, c) -= acc[a][b][v];
             }
+}
+
+
+// Large symmetric update of the lower frontal matrix:
+//   F[r][c] -= sum_{k in [k0, k1)} L[r][k] L[c][k]   for r >= c, c in [cs, ce), r in [cs, m)
+// Used for the trailing update of the pivot columns (k1 - k0 = NBO, ce = p) and for the
+// Schur complement of the update block in one pass (k0 = 0, k1 = p, cs = p, ce = m), so that
+// each element of U is read and written once per factorization.
+// (2 WM)^2 tiles (blockIdx.x = row tile, blockIdx.y = column tile, blockIdx.z = front); each of
+// the 4 wavefronts owns a WM x WM quarter as MFMA 16x16x4 blocks. K is streamed in chunks of KC
+// through double-buffered LDS with the next chunk's global loads in flight during the MFMAs,
+// and the LDS fragments of the next k-step are read before the current k-step's MFMAs.
+// The MFMA computes the transposed block (column operand first), so a lane's accumulators map
+// to 16 consecutive rows of C and the read-modify-write of C is coalesced along columns.
+constexpr int UPAD = 4;
+
+template <typename T, int WM, int KC>
+__global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                int k0, int k1, int mode, T* __restrict__ L,
+                                                T* __restrict__ U) {
+    constexpr int TM = 2 * WM, NA = WM / 16, E = TM * KC / 256, KSTEP = 256 / TM;
+    const DevFront Fr = F[flist[blockIdx.z]];
+    const int p = Fr.p, m = Fr.m;
+    // mode 0: pivot-column trailing update, columns [k1, p); mode 1: update block, columns [p, m)
+    const int kend = min(k1, p);
+    if (kend <= k0) return;
+    const int cs = mode == 0 ? kend : p;
+    const int ce = mode == 0 ? p : m;
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (ti < tj) return;
+    const int R0 = cs + TM * ti, C0 = cs + TM * tj;
+    if (R0 >= m || C0 >= ce) return;
+    __shared__ T Rs[2][KC][TM + UPAD];
+    __shared__ T Cs[2][KC][TM + UPAD];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int wr = (wave >> 1) * WM, wc = (wave & 1) * WM;
+    const bool skip = (ti == tj && wr < wc);  // strictly upper quarter of a diagonal tile
+    const int li = lane & 15, lk = lane >> 4;
+    const T* Lf = L + Fr.loff;
+    // loader: element e = t + 256 j (j < E) -> row e % TM, k e / TM
+    const int lrow = t % TM, lkk = t / TM;
+    const bool rin = R0 + lrow < m, cin = C0 + lrow < ce;
+    T rreg[E], creg[E];
+    auto load = [&](int kc) {
+        for (int j = 0; j < E; ++j) {
+            const int k = kc + lkk + KSTEP * j;
+            const bool kin = k < kend;
+            const T* col = Lf + int64_t(k) * m;
+            rreg[j] = (kin && rin) ? col[R0 + lrow] : T(0);
+            creg[j] = (kin && cin) ? col[C0 + lrow] : T(0);
+        }
+    };
+    auto stash = [&](int b) {
+        for (int j = 0; j < E; ++j) {
+            Rs[b][lkk + KSTEP * j][lrow] = rreg[j];
+            Cs[b][lkk + KSTEP * j][lrow] = creg[j];
+        }
+    };
+    typename Mfma<T>::acc_t acc[NA][NA];
+    for (int a = 0; a < NA; ++a)
+        for (int b = 0; b < NA; ++b)
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
+    const int nchunk = (kend - k0 + KC - 1) / KC;
+    load(k0);
+    stash(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nchunk;
+        if (more) load(k0 + (ch + 1) * KC);
+        if (!skip) {
+            T cf[NA], rf[NA];
+            for (int a = 0; a < NA; ++a) cf[a] = Cs[buf][lk][wc + 16 * a + li];
+            for (int b = 0; b < NA; ++b) rf[b] = Rs[buf][lk][wr + 16 * b + li];
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 4) {
+                T cn[NA], rn[NA];
+                if (kk + 4 < KC) {
+                    for (int a = 0; a < NA; ++a) cn[a] = Cs[buf][kk + 4 + lk][wc + 16 * a + li];
+                    for (int b = 0; b < NA; ++b) rn[b] = Rs[buf][kk + 4 + lk][wr + 16 * b + li];
+                }
+                for (int a = 0; a < NA; ++a)
+                    for (int b = 0; b < NA; ++b) acc[a][b] = Mfma<T>::op(cf[a], rf[b], acc[a][b]);
+                if (kk + 4 < KC) {
+                    for (int a = 0; a < NA; ++a) cf[a] = cn[a];
+                    for (int b = 0; b < NA; ++b) rf[b] = rn[b];
+                }
+            }
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+    }
+    if (skip) return;
+    // acc[a][b][v] = F[R0 + wr + 16b + li][C0 + wc + 16a + row(lk, v)]
+    const int u = m - p;
+    for (int a = 0; a < NA; ++a)
+        for (int v = 0; v < 4; ++v) {
+            const int c = C0 + wc + 16 * a + Mfma<T>::row(lk, v);
+            if (c >= ce) continue;
+            T* colp = c < p ? L + Fr.loff + int64_t(c) * m : U + Fr.uoff + int64_t(c - p) * u - p;
+            for (int b = 0; b < NA; ++b) {
+                const int r = R0 + wr + 16 * b + li;
+                if (r < m && r >= c) colp[r] -= acc[a][b][v];
+            }
+        }
 }
 
 // ------------------------------------------------------------------ triangular sweeps
@@ -696,6 +793,9 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
     L_.alloc(std::max<int64_t>(S.lsize, 1));
     U_.alloc(std::max<int64_t>(S.usize, 1));
     info_.alloc(1);
+    panel_done_.alloc(std::max(1, S.nf));
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(S.nf, 256)), dim3(256), 0, stream, panel_done_.p,
+                       int64_t(std::max(1, S.nf)), 0);
     // levels with fronts sorted by decreasing pivot count (active prefix per panel step)
     std::vector<int> sorted;
     level_sorted_off_.assign(S.nlevels + 1, 0);
@@ -726,6 +826,18 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
     EMTD_HIP(hipStreamSynchronize(stream));
 }
 
+// Launches k_update over cnt fronts for a target block of at most rows x cols: 64x64 tiles
+// with K chunks of 128 bytes per row (16 FP64 / 32 FP32 values), which keeps 4 workgroups
+// resident per CU (measured faster than 128x128 tiles at one workgroup per CU).
+template <typename T>
+void GpuCholesky<T>::update(const int* fz, int cnt, int k0, int k1, int mode, int rows, int cols) {
+    if (rows <= 0 || cols <= 0 || cnt <= 0) return;
+    constexpr int KC = 128 / int(sizeof(T));
+    const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
+    hipLaunchKernelGGL((k_update<T, 32, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1, mode,
+                       L_.p, U_.p);
+}
+
 template <typename T>
 void GpuCholesky<T>::numeric() {
     const Symbolic& S = S_;
@@ -751,26 +863,27 @@ void GpuCholesky<T>::numeric() {
                 const int cnt = std::min(65535, nact - nz);
                 const int* fz = dfl + nz;
                 for (int ks = kb; ks < std::min(kb + NBO, maxp); ks += NB) {
-                    hipLaunchKernelGGL(k_potrf<T>, dim3(cnt), dim3(256), 0, stream_, fronts_.p, fz, ks, L_.p, info_.p);
                     const int rem = mm - ks - 1;
+                    hipLaunchKernelGGL(k_panel<T>, dim3(std::max(1, (rem + 63) / 64), cnt), dim3(64), 0, stream_,
+                                       fronts_.p, fz, ks, L_.p, info_.p, panel_done_.p);
                     if (rem <= 0) continue;
-                    hipLaunchKernelGGL(k_trsm<T>, dim3((rem + 63) / 64, cnt), dim3(64), 0, stream_, fronts_.p, fz, ks,
-                                       L_.p);
                     // inside the outer block: columns [ks+nb, kb+NBO)
                     const int pc = (kb + NBO) - (ks + 1);
                     if (pc > 0)
                         hipLaunchKernelGGL(k_syrk<T>, dim3((rem + TILE - 1) / TILE, (pc + TILE - 1) / TILE, cnt),
-                                           dim3(256), 0, stream_, fronts_.p, fz, kb, ks, NB, 0, L_.p, U_.p);
+                                           dim3(256), 0, stream_, fronts_.p, fz, kb, ks, NB, L_.p, U_.p);
                 }
-                // trailing matrix: columns [kb+nbo, m) with K = nbo
-                const int rem = mm - kb - 1;
-                if (rem > 0) {
-                    const int t = (rem + TILE - 1) / TILE;
-                    hipLaunchKernelGGL(k_syrk<T>, dim3(t, t, cnt), dim3(256), 0, stream_, fronts_.p, fz, kb, kb, NBO, 1,
-                                       L_.p, U_.p);
-                }
+                // trailing pivot columns [kb+NBO, p) with K = NBO, rows to m
+                int mp = 0;
+                for (int i = 0; i < nact; ++i) mp = std::max(mp, S.pcount[fl[nz + i]]);
+                update(fz, cnt, kb, kb + NBO, 0, mm - (kb + NBO), mp - (kb + NBO));
             }
         }
+        // update block of every front: U -= L21 L21^T with K = p, one pass
+        int mu = 0;
+        for (int f : fl) mu = std::max(mu, S.mcount[f] - S.pcount[f]);
+        for (int nz = 0; nz < int(fl.size()); nz += 65535)
+            update(dfl + nz, std::min(65535, int(fl.size()) - nz), 0, INT_MAX, 1, mu, mu);
     }
     EMTD_HIP(hipGetLastError());
 }

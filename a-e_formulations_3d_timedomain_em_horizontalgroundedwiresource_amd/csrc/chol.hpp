@@ -51,6 +51,7 @@ class GpuCholesky {
 
   private:
     void numeric();
+    void update(const int* fz, int cnt, int k0, int k1, int mode, int rows, int cols);
     void prepare_big_levels();
     void prepare_inverses();
     void invert_diagonal_blocks();
@@ -65,7 +66,7 @@ class GpuCholesky {
     std::vector<std::vector<int>> level_sorted_;  // fronts of a level by decreasing p
     DevBuf<int> level_sorted_dev_;
     std::vector<int> level_sorted_off_;
-    DevBuf<int> info_;
+    DevBuf<int> info_, panel_done_;
     int wrhs_ = 0;
     int big_level_fronts_ = 1024;
     bool big_ready_ = false;
