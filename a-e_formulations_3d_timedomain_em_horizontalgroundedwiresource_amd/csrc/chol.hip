@@ -32,7 +32,7 @@ namespace emtd {
 namespace {
 
 constexpr int NB = 32;    // panel width of the blocked factorization
-constexpr int TILE = 64;  // trailing-update tile
+constexpr int NBO = 128;  // outer block of the factorization
 constexpr int SB = 64;    // column block of the triangular sweeps
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -110,12 +110,14 @@ __global__ void k_extend_add(const DevFront* __restrict__ F, const int2* __restr
 }
 
 // One panel step of the blocked factorization, for the columns [kb, kb+nb) of each active
-// front: every workgroup factors the nb x nb diagonal block in LDS (the same arithmetic in
-// every workgroup, so all copies agree), then solves X * L_kk^T = B for its 64 rows below the
-// diagonal block. The diagonal block may only be overwritten once every workgroup of the front
-// has read it: each workgroup counts itself in done[front] after its reads, and the last one
-// stores the factored block, reports the first non-positive pivot (as a global index) through
-// atomicMin(info) and resets the counter for the next panel.
+// front. Every workgroup (one wavefront, 64 panel rows) factors the nb x nb diagonal block
+// itself (the same arithmetic in every workgroup, so all copies agree): lanes 0..nb-1 hold the
+// block's rows in registers, and column j of the factor is shared through LDS, from which
+// every lane also applies it to its panel row (right-looking X * L_kk^T = B). The diagonal
+// block may only be overwritten once every workgroup of the front has read it: each workgroup
+// counts itself in done[front] after its reads, and the last one stores the factored block,
+// reports the first non-positive pivot (as a global index) through atomicMin(info) and resets
+// the counter for the next panel.
 template <typename T>
 __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
                                               T* __restrict__ L, int* __restrict__ info, int* __restrict__ done) {
@@ -126,52 +128,62 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
     const int r0 = kb + nb + 64 * blockIdx.x;
     if (blockIdx.x > 0 && r0 >= Fr.m) return;
     const int nwg = max(1, (Fr.m - kb - nb + 63) / 64);
+    __shared__ T colj[NB];
     __shared__ int s_last;
-    __shared__ T A[NB][NB + 1];
-    __shared__ T X[64][NB + 1];
-    T* col = L + Fr.loff + int64_t(kb) * Fr.m;
     const int t = threadIdx.x;
-    for (int j = 0; j < nb; ++j)
-        if (t < nb) A[t][j] = t >= j ? col[int64_t(j) * Fr.m + kb + t] : T(0);
+    const int64_t m = Fr.m;
+    T* col = L + Fr.loff + int64_t(kb) * m;
     const int r = r0 + t;
-    if (r < Fr.m)
-        for (int j = 0; j < nb; ++j) X[t][j] = col[int64_t(j) * Fr.m + r];
-    __syncthreads();
-    if (t == 0) s_last = atomicAdd(done + f, 1) == nwg - 1;
-    int bad = -1;
-    const int l = t & 31, half = t >> 5;
-    for (int j = 0; j < nb; ++j) {
-        T d = A[j][j];
-        if (!(d > T(0))) {
-            if (bad < 0) bad = j;
-            d = T(1);
-        }
-        d = sqrt(d);
-        __syncthreads();
-        if (t == 0) A[j][j] = d;
-        if (t > j && t < nb) A[t][j] /= d;
-        __syncthreads();
-        if (l > j && l < nb)
-            for (int i = j + 1 + half; i < nb; i += 2)
-                if (l <= i) A[i][l] -= A[i][j] * A[l][j];
-        __syncthreads();
+    const bool diag = t < nb, row = r < Fr.m;
+    T a[NB], x[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        a[j] = (diag && j < nb && t >= j) ? col[j * m + kb + t] : T(0);
+        x[j] = (row && j < nb) ? col[j * m + r] : T(0);
     }
+    int bad = -1;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < nb) {
+            if (t == j) colj[j] = a[j];
+            __syncthreads();
+            T d = colj[j];
+            if (!(d > T(0))) {
+                if (bad < 0) bad = j;
+                d = T(1);
+            }
+            d = sqrt(d);
+            if (t == j) a[j] = d;
+            if (t > j) a[j] /= d;
+            __syncthreads();
+            if (diag && t >= j) colj[t] = a[j];  // column j of the factor, rows j..nb-1
+            __syncthreads();
+            x[j] /= d;
+#pragma unroll
+            for (int l = j + 1; l < NB; ++l) {
+                if (l < nb) {
+                    const T c = colj[l];
+                    a[l] -= a[j] * c;
+                    x[l] -= x[j] * c;
+                }
+            }
+        }
+    }
+    if (t == 0) s_last = atomicAdd(done + f, 1) == nwg - 1;
+    __syncthreads();
     if (s_last) {
-        for (int j = 0; j < nb; ++j)
-            if (t < nb && t >= j) col[int64_t(j) * Fr.m + kb + t] = A[t][j];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (diag && j < nb && t >= j) col[j * m + kb + t] = a[j];
         if (t == 0) {
             if (bad >= 0) atomicMin(info, Fr.fbeg + kb + bad);
             done[f] = 0;
         }
     }
-    if (r < Fr.m) {
-        for (int j = 0; j < nb; ++j) {
-            T v = X[t][j];
-            for (int k = 0; k < j; ++k) v -= X[t][k] * A[j][k];
-            X[t][j] = v / A[j][j];
-        }
-        for (int j = 0; j < nb; ++j) col[int64_t(j) * Fr.m + r] = X[t][j];
-    }
+    if (row)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb) col[j * m + r] = x[j];
 }
 
 template <typename T>
@@ -196,75 +208,12 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
-// Symmetric rank-k update inside the current outer block: with the factored columns
-// [k0, k0+kd), kd = min(kmax, p-k0),  F[r][c] -= sum_k L[r][k] L[c][k]  for r >= c in the
-// target columns [k0+kd, min(kb+NBO, p)). 64x64 tiles (blockIdx.x = row tile, blockIdx.y =
-// column tile, blockIdx.z = front); 4 wavefronts own a 32x32 quarter each as 2x2 MFMA 16x16x4
-// blocks (lane layouts in Mfma<T>); K is streamed through LDS in chunks of 32.
-constexpr int NBO = 128;  // outer block of the factorization
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_syrk(const DevFront* __restrict__ F, const int* __restrict__ flist, int kb,
-                                              int k0, int kmax, T* __restrict__ L, T* __restrict__ U) {
-    const DevFront Fr = F[flist[blockIdx.z]];
-    const int kd = min(kmax, Fr.p - k0);
-    if (kd <= 0) return;
-    const int cs = k0 + kd;
-    const int ce = min(kb + NBO, Fr.p);
-    if (cs >= ce) return;
-    const int ti = blockIdx.x, tj = blockIdx.y;
-    if (ti < tj) return;
-    const int R0 = cs + TILE * ti, C0 = cs + TILE * tj;
-    if (R0 >= Fr.m || C0 >= ce) return;
-    constexpr int KC = 32, KP = KC + 4;
-    __shared__ T As[TILE][KP];
-    __shared__ T Bs[TILE][KP];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const bool skip = (ti == tj && wr + 31 < wc);  // strictly upper quarter of a diagonal tile
-    typename Mfma<T>::acc_t acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
-    const int li = lane & 15, lk = lane >> 4;
-    for (int kc = 0; kc < kd; kc += KC) {
-        const int nk = min(KC, kd - kc);
-        const T* pan = L + Fr.loff + int64_t(k0 + kc) * Fr.m;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < TILE * KC; idx += blockDim.x) {
-            const int i = idx % TILE, k = idx / TILE;
-            const int ra = R0 + i, rb = C0 + i;
-            As[i][k] = (k < nk && ra < Fr.m) ? pan[int64_t(k) * Fr.m + ra] : T(0);
-            Bs[i][k] = (k < nk && rb < ce) ? pan[int64_t(k) * Fr.m + rb] : T(0);
-        }
-        __syncthreads();
-        if (!skip) {
-            const int nk4 = (nk + 3) & ~3;
-            for (int kk = 0; kk < nk4; kk += 4) {
-                T av[2], bv[2];
-                for (int a = 0; a < 2; ++a) av[a] = As[wr + 16 * a + li][kk + lk];
-                for (int b = 0; b < 2; ++b) bv[b] = Bs[wc + 16 * b + li][kk + lk];
-                for (int a = 0; a < 2; ++a)
-                    for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
-            }
-        }
-    }
-    if (skip) return;
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) {
-                const int r = R0 + wr + 16 * a + Mfma<T>::row(lk, v);
-                const int c = C0 + wc + 16 * b + li;
-                if (r < Fr.m && c < ce && r >= c) *front_elem(Fr, L, U, r, c) -= acc[a][b][v];
-            }
-}
-
-
-// Large symmetric update of the lower frontal matrix:
+// Symmetric update of the lower frontal matrix:
 //   F[r][c] -= sum_{k in [k0, k1)} L[r][k] L[c][k]   for r >= c, c in [cs, ce), r in [cs, m)
-// Used for the trailing update of the pivot columns (k1 - k0 = NBO, ce = p) and for the
-// Schur complement of the update block in one pass (k0 = 0, k1 = p, cs = p, ce = m), so that
-// each element of U is read and written once per factorization.
+// with cs = k1 and ce = min(cmax, p) for the pivot columns (inside the current outer block,
+// cmax = kb + NBO, and the trailing pivot columns, cmax = INT_MAX), and, for cmax < 0, the
+// Schur complement of the update block in one pass: U = -L21 L21^T (k0 = 0, k1 = p, cs = p,
+// ce = m), stored without reading U; the children's contributions to U are added afterwards.
 // (2 WM)^2 tiles (blockIdx.x = row tile, blockIdx.y = column tile, blockIdx.z = front); each of
 // the 4 wavefronts owns a WM x WM quarter as MFMA 16x16x4 blocks. K is streamed in chunks of KC
 // through double-buffered LDS with the next chunk's global loads in flight during the MFMAs,
@@ -275,16 +224,17 @@ constexpr int UPAD = 4;
 
 template <typename T, int WM, int KC>
 __global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                int k0, int k1, int mode, T* __restrict__ L,
+                                                int k0, int k1, int cmax, T* __restrict__ L,
                                                 T* __restrict__ U) {
     constexpr int TM = 2 * WM, NA = WM / 16, E = TM * KC / 256, KSTEP = 256 / TM;
     const DevFront Fr = F[flist[blockIdx.z]];
     const int p = Fr.p, m = Fr.m;
-    // mode 0: pivot-column trailing update, columns [k1, p); mode 1: update block, columns [p, m)
+    // cmax >= 0: pivot columns [k1, min(cmax, p)) -= ...;  cmax < 0: update block U = -L21 L21^T
+    const bool ublock = cmax < 0;
     const int kend = min(k1, p);
     if (kend <= k0) return;
-    const int cs = mode == 0 ? kend : p;
-    const int ce = mode == 0 ? p : m;
+    const int cs = ublock ? p : kend;
+    const int ce = ublock ? m : min(cmax, p);
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (ti < tj) return;
     const int R0 = cs + TM * ti, C0 = cs + TM * tj;
@@ -359,7 +309,12 @@ __global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, 
             T* colp = c < p ? L + Fr.loff + int64_t(c) * m : U + Fr.uoff + int64_t(c - p) * u - p;
             for (int b = 0; b < NA; ++b) {
                 const int r = R0 + wr + 16 * b + li;
-                if (r < m && r >= c) colp[r] -= acc[a][b][v];
+                if (r < m && r >= c) {
+                    if (ublock)
+                        colp[r] = -acc[a][b][v];
+                    else
+                        colp[r] -= acc[a][b][v];
+                }
             }
         }
 }
@@ -808,9 +763,13 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
     }
     level_sorted_dev_ = to_device(sorted, stream);
     // extend-add work items per (level, pass)
+    // Columns of a child's update matrix that land in the parent's pivot columns are added
+    // before the parent is factored; the others land in the parent's update block, which the
+    // Schur-complement kernel overwrites, so they are added after it.
     const int P = std::max(1, S.max_children);
-    std::vector<int2> items;
+    std::vector<int2> items, uitems;
     ea_ptr_.assign(S.nlevels * P + 1, 0);
+    eau_ptr_.assign(S.nlevels * P + 1, 0);
     for (int l = 0; l < S.nlevels; ++l)
         for (int k = 0; k < P; ++k) {
             for (int li = S.level_ptr[l]; li < S.level_ptr[l + 1]; ++li) {
@@ -818,11 +777,16 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
                 if (S.child_ptr[f] + k >= S.child_ptr[f + 1]) continue;
                 const int c = S.children[S.child_ptr[f] + k];
                 const int uc = S.mcount[c] - S.pcount[c];
-                for (int b = 0; b < uc; ++b) items.push_back(make_int2(c, b));
+                const int* rm = S.relmap.data() + S.relmap_ptr[c];
+                for (int b = 0; b < uc; ++b) (rm[b] < S.pcount[f] ? items : uitems).push_back(make_int2(c, b));
             }
             ea_ptr_[l * P + k + 1] = int(items.size());
+            eau_ptr_[l * P + k + 1] = int(uitems.size());
         }
+    if (items.empty()) items.push_back(make_int2(0, 0));
+    if (uitems.empty()) uitems.push_back(make_int2(0, 0));
     ea_items_ = to_device(items, stream);
+    eau_items_ = to_device(uitems, stream);
     EMTD_HIP(hipStreamSynchronize(stream));
 }
 
@@ -830,26 +794,30 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
 // with K chunks of 128 bytes per row (16 FP64 / 32 FP32 values), which keeps 4 workgroups
 // resident per CU (measured faster than 128x128 tiles at one workgroup per CU).
 template <typename T>
-void GpuCholesky<T>::update(const int* fz, int cnt, int k0, int k1, int mode, int rows, int cols) {
+void GpuCholesky<T>::update(const int* fz, int cnt, int k0, int k1, int cmax, int rows, int cols) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
     const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
-    hipLaunchKernelGGL((k_update<T, 32, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1, mode,
+    hipLaunchKernelGGL((k_update<T, 32, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1, cmax,
                        L_.p, U_.p);
+}
+
+template <typename T>
+void GpuCholesky<T>::extend_add(int l, const std::vector<int>& ptr, const int2* items) {
+    const int P = std::max(1, S_.max_children);
+    for (int k = 0; k < P; ++k) {
+        const int b = ptr[l * P + k], e = ptr[l * P + k + 1];
+        if (e > b)
+            hipLaunchKernelGGL(k_extend_add<T>, dim3((e - b + 3) / 4), dim3(256), 0, stream_, fronts_.p, items + b,
+                               e - b, relmap_.p, L_.p, U_.p);
+    }
 }
 
 template <typename T>
 void GpuCholesky<T>::numeric() {
     const Symbolic& S = S_;
-    const int P = std::max(1, S.max_children);
     for (int l = 0; l < S.nlevels; ++l) {
-        if (l > 0)
-            for (int k = 0; k < P; ++k) {
-                const int b = ea_ptr_[l * P + k], e = ea_ptr_[l * P + k + 1];
-                if (e > b)
-                    hipLaunchKernelGGL(k_extend_add<T>, dim3((e - b + 3) / 4), dim3(256), 0, stream_,
-                                       fronts_.p, ea_items_.p + b, e - b, relmap_.p, L_.p, U_.p);
-            }
+        if (l > 0) extend_add(l, ea_ptr_, ea_items_.p);
         const std::vector<int>& fl = level_sorted_[l];
         const int* dfl = level_sorted_dev_.p + level_sorted_off_[l];
         const int maxp = S.pcount[fl[0]];
@@ -857,8 +825,11 @@ void GpuCholesky<T>::numeric() {
         for (int kb = 0; kb < maxp; kb += NBO) {
             while (nact > 0 && S.pcount[fl[nact - 1]] <= kb) --nact;
             if (nact == 0) break;
-            int mm = 0;
-            for (int i = 0; i < nact; ++i) mm = std::max(mm, S.mcount[fl[i]]);
+            int mm = 0, mp = 0;
+            for (int i = 0; i < nact; ++i) {
+                mm = std::max(mm, S.mcount[fl[i]]);
+                mp = std::max(mp, S.pcount[fl[i]]);
+            }
             for (int nz = 0; nz < nact; nz += 65535) {
                 const int cnt = std::min(65535, nact - nz);
                 const int* fz = dfl + nz;
@@ -866,24 +837,19 @@ void GpuCholesky<T>::numeric() {
                     const int rem = mm - ks - 1;
                     hipLaunchKernelGGL(k_panel<T>, dim3(std::max(1, (rem + 63) / 64), cnt), dim3(64), 0, stream_,
                                        fronts_.p, fz, ks, L_.p, info_.p, panel_done_.p);
-                    if (rem <= 0) continue;
-                    // inside the outer block: columns [ks+nb, kb+NBO)
-                    const int pc = (kb + NBO) - (ks + 1);
-                    if (pc > 0)
-                        hipLaunchKernelGGL(k_syrk<T>, dim3((rem + TILE - 1) / TILE, (pc + TILE - 1) / TILE, cnt),
-                                           dim3(256), 0, stream_, fronts_.p, fz, kb, ks, NB, L_.p, U_.p);
+                    // inside the outer block: columns [ks+NB, kb+NBO) with K = NB
+                    update(fz, cnt, ks, ks + NB, kb + NBO, rem, std::min(kb + NBO, mp) - (ks + 1));
                 }
                 // trailing pivot columns [kb+NBO, p) with K = NBO, rows to m
-                int mp = 0;
-                for (int i = 0; i < nact; ++i) mp = std::max(mp, S.pcount[fl[nz + i]]);
-                update(fz, cnt, kb, kb + NBO, 0, mm - (kb + NBO), mp - (kb + NBO));
+                update(fz, cnt, kb, kb + NBO, INT_MAX, mm - (kb + NBO), mp - (kb + NBO));
             }
         }
-        // update block of every front: U -= L21 L21^T with K = p, one pass
+        // update block of every front: U = -L21 L21^T with K = p, one pass
         int mu = 0;
         for (int f : fl) mu = std::max(mu, S.mcount[f] - S.pcount[f]);
         for (int nz = 0; nz < int(fl.size()); nz += 65535)
-            update(dfl + nz, std::min(65535, int(fl.size()) - nz), 0, INT_MAX, 1, mu, mu);
+            update(dfl + nz, std::min(65535, int(fl.size()) - nz), 0, INT_MAX, -1, mu, mu);
+        if (l > 0) extend_add(l, eau_ptr_, eau_items_.p);
     }
     EMTD_HIP(hipGetLastError());
 }
@@ -899,8 +865,6 @@ void GpuCholesky<T>::factorize_system(const int64_t* d_dest_off, const int64_t* 
     const int init = INT_MAX;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, stream_, info_.p, int64_t(1), init);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(S_.lsize, 256)), dim3(256), 0, stream_, L_.p, S_.lsize, T(0));
-    if (S_.usize)
-        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(S_.usize, 256)), dim3(256), 0, stream_, U_.p, S_.usize, T(0));
     if (nnz_low)
         hipLaunchKernelGGL(k_scatter_system<T>, dim3(unsigned((nnz_low + 255) / 256)), dim3(256), 0, stream_,
                            d_dest_off, d_b_ggl, d_c_ggl, nnz_low, mult, L_.p);
@@ -927,8 +891,6 @@ void GpuCholesky<T>::factorize_values(const int64_t* d_dest_off, const int64_t* 
     const int init = INT_MAX;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, stream_, info_.p, int64_t(1), init);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(S_.lsize, 256)), dim3(256), 0, stream_, L_.p, S_.lsize, T(0));
-    if (S_.usize)
-        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(S_.usize, 256)), dim3(256), 0, stream_, U_.p, S_.usize, T(0));
     if (nnz_low)
         hipLaunchKernelGGL(k_scatter_values<T>, dim3(unsigned((nnz_low + 255) / 256)), dim3(256), 0, stream_,
                            d_dest_off, d_ggl, nnz_low, L_.p);

@@ -51,7 +51,8 @@ class GpuCholesky {
 
   private:
     void numeric();
-    void update(const int* fz, int cnt, int k0, int k1, int mode, int rows, int cols);
+    void update(const int* fz, int cnt, int k0, int k1, int cmax, int rows, int cols);
+    void extend_add(int l, const std::vector<int>& ptr, const int2* items);
     void prepare_big_levels();
     void prepare_inverses();
     void invert_diagonal_blocks();
@@ -61,8 +62,8 @@ class GpuCholesky {
     DevBuf<double> W_;
     DevBuf<DevFront> fronts_;
     DevBuf<int> rows_, children_, relmap_, level_fronts_;
-    DevBuf<int2> ea_items_;             // extend-add work items (child, column)
-    std::vector<int> ea_ptr_;           // per (level, pass) range into ea_items_
+    DevBuf<int2> ea_items_, eau_items_;   // extend-add work items (child, column): into L / into U
+    std::vector<int> ea_ptr_, eau_ptr_;   // per (level, pass) ranges into the item lists
     std::vector<std::vector<int>> level_sorted_;  // fronts of a level by decreasing p
     DevBuf<int> level_sorted_dev_;
     std::vector<int> level_sorted_off_;
