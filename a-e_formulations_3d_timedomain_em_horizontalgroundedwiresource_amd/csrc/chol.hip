@@ -214,19 +214,23 @@ struct Mfma<float> {
 // cmax = kb + NBO, and the trailing pivot columns, cmax = INT_MAX), and, for cmax < 0, the
 // Schur complement of the update block in one pass: U = -L21 L21^T (k0 = 0, k1 = p, cs = p,
 // ce = m), stored without reading U; the children's contributions to U are added afterwards.
-// (2 WM)^2 tiles (blockIdx.x = row tile, blockIdx.y = column tile, blockIdx.z = front); each of
-// the 4 wavefronts owns a WM x WM quarter as MFMA 16x16x4 blocks. K is streamed in chunks of KC
-// through double-buffered LDS with the next chunk's global loads in flight during the MFMAs,
-// and the LDS fragments of the next k-step are read before the current k-step's MFMAs.
-// The MFMA computes the transposed block (column operand first), so a lane's accumulators map
-// to 16 consecutive rows of C and the read-modify-write of C is coalesced along columns.
+// TM x TM tiles (blockIdx.x = row tile, blockIdx.y = column tile, blockIdx.z = front) over
+// WV x WV wavefronts, each owning a (TM/WV)^2 block as MFMA 16x16x4 blocks. Many waves with a
+// few accumulators each keep the FP64 matrix pipes busier than few waves with many (measured
+// on gfx950: 128x128 tiles over 16 waves of 32x32 beat 4 waves of 64x64 by 1.7x). K is
+// streamed in chunks of KC through double-buffered LDS with the next chunk's global loads in
+// flight during the MFMAs, and the LDS fragments of the next k-step are read before the
+// current k-step's MFMAs. The MFMA computes the transposed block (column operand first), so a
+// lane's accumulators map to 16 consecutive rows of C and the read-modify-write of C is
+// coalesced along columns.
 constexpr int UPAD = 4;
 
-template <typename T, int WM, int KC>
-__global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                int k0, int k1, int cmax, T* __restrict__ L,
-                                                T* __restrict__ U) {
-    constexpr int TM = 2 * WM, NA = WM / 16, E = TM * KC / 256, KSTEP = 256 / TM;
+template <typename T, int TM, int WV, int KC>
+__global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                        int k0, int k1, int cmax, T* __restrict__ L,
+                                                        T* __restrict__ U) {
+    constexpr int NT = 64 * WV * WV, WT = TM / WV, NA = WT / 16, E = TM * KC / NT, KS = NT / TM;
+    static_assert(NT % TM == 0 && (TM * KC) % NT == 0 && WT % 16 == 0, "tile shape");
     const DevFront Fr = F[flist[blockIdx.z]];
     const int p = Fr.p, m = Fr.m;
     // cmax >= 0: pivot columns [k1, min(cmax, p)) -= ...;  cmax < 0: update block U = -L21 L21^T
@@ -242,17 +246,17 @@ __global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, 
     __shared__ T Rs[2][KC][TM + UPAD];
     __shared__ T Cs[2][KC][TM + UPAD];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int wr = (wave >> 1) * WM, wc = (wave & 1) * WM;
-    const bool skip = (ti == tj && wr < wc);  // strictly upper quarter of a diagonal tile
+    const int wr = (wave / WV) * WT, wc = (wave % WV) * WT;
+    const bool skip = (ti == tj && wr < wc);  // strictly upper block of a diagonal tile
     const int li = lane & 15, lk = lane >> 4;
     const T* Lf = L + Fr.loff;
-    // loader: element e = t + 256 j (j < E) -> row e % TM, k e / TM
+    // loader: element e = t + NT j (j < E) -> row e % TM, k e / TM
     const int lrow = t % TM, lkk = t / TM;
     const bool rin = R0 + lrow < m, cin = C0 + lrow < ce;
     T rreg[E], creg[E];
     auto load = [&](int kc) {
         for (int j = 0; j < E; ++j) {
-            const int k = kc + lkk + KSTEP * j;
+            const int k = kc + lkk + KS * j;
             const bool kin = k < kend;
             const T* col = Lf + int64_t(k) * m;
             rreg[j] = (kin && rin) ? col[R0 + lrow] : T(0);
@@ -261,8 +265,8 @@ __global__ void __launch_bounds__(256) k_update(const DevFront* __restrict__ F, 
     };
     auto stash = [&](int b) {
         for (int j = 0; j < E; ++j) {
-            Rs[b][lkk + KSTEP * j][lrow] = rreg[j];
-            Cs[b][lkk + KSTEP * j][lrow] = creg[j];
+            Rs[b][lkk + KS * j][lrow] = rreg[j];
+            Cs[b][lkk + KS * j][lrow] = creg[j];
         }
     };
     typename Mfma<T>::acc_t acc[NA][NA];
@@ -790,16 +794,22 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
     EMTD_HIP(hipStreamSynchronize(stream));
 }
 
-// Launches k_update over cnt fronts for a target block of at most rows x cols: 64x64 tiles
-// with K chunks of 128 bytes per row (16 FP64 / 32 FP32 values), which keeps 4 workgroups
-// resident per CU (measured faster than 128x128 tiles at one workgroup per CU).
+// Launches k_update over cnt fronts for a target block of at most rows x cols: 128x128 tiles
+// over 16 waves for large blocks, 64x64 tiles over 4 waves for small ones; K chunks of 128
+// bytes per row (16 FP64 / 32 FP32 values).
 template <typename T>
 void GpuCholesky<T>::update(const int* fz, int cnt, int k0, int k1, int cmax, int rows, int cols) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
-    const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
-    hipLaunchKernelGGL((k_update<T, 32, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1, cmax,
-                       L_.p, U_.p);
+    if (cols >= 256) {
+        const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
+        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), 0, stream_, fronts_.p, fz, k0, k1,
+                           cmax, L_.p, U_.p);
+    } else {
+        const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
+        hipLaunchKernelGGL((k_update<T, 64, 2, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1,
+                           cmax, L_.p, U_.p);
+    }
 }
 
 template <typename T>
