@@ -9,8 +9,10 @@ receiver evaluation; the first step of each decade also refactorizes the system 
 
 Prints ONE JSON line. ``value`` = timed steps / wall time of the timed region (inputs and
 factor storage already resident in HBM). The roofline object covers the dominant per-step
-kernels (the triangular sweeps): algorithmic bytes = 8 bytes x nnz(L) read by each of the
-two sweeps, per right-hand side, divided by their GPU time measured with HIP events.
+kernels (one triangular solve = the forward and backward sweep kernels): algorithmic bytes =
+8 bytes x nnz(L) read by each of the two sweeps, per right-hand side, divided by the solve's GPU
+time measured with HIP events on the solver's stream; ``traffic`` is the HBM bytes per solve
+measured with rocprofv3 counters on this workload (profiles/sweeps_config3_traffic.md).
 """
 from __future__ import annotations
 
@@ -73,6 +75,23 @@ def cpu_baseline(cells=(20, 20, 16), steps_per_decade=20):
                        f"assembly + 1 factorization + {nsteps} steps + receivers, {dt:.1f} s")}
 
 
+def measured_traffic(args, cells):
+    """HBM bytes per solve measured with rocprofv3 FETCH_SIZE/WRITE_SIZE on this workload (the
+    summary under profiles/ gives the commands and the copy-kernel calibration), or None when
+    the run is not the profiled workload."""
+    if args.config != 3 or cells is not None or args.precision != "fp64":
+        return None
+    path = os.path.join(ROOT, "profiles", "sweeps_config3_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_solve:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +101,19 @@ def main():
     ap.add_argument("--cells", type=str, default=None, help="override cells, e.g. 40x40x30")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (3 times), for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_calibration} bytes 3 times")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               f"the multi-GPU design is described in DESIGN.md and not measured.", file=sys.stderr)
@@ -141,8 +172,10 @@ def main():
                        "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
                        "setup_s": round(ci.setup_seconds, 2)},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                         "kernel": "triangular sweeps k_fwd+k_bwd, one solve per step",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic(args, cells),
+                         "kernel": ("one triangular solve (forward + backward sweep kernels) per step; "
+                                    "bytes = 2 sweeps x %d B x nnz(L) x sources" % elem),
                          "bytes_per_launch": bytes_solve},
         }
         c.close()
