@@ -546,10 +546,12 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     double* v = W + Fr.woff;
     int* fl = flags + foff[it.x];
     double* yb = ybuf + int64_t(foff[it.x]) * 64;
-    // this thread's 16 entries of inv(L_bb) (row lr, columns 16q..16q+15), loaded before any wait
+    // loaded before any wait: this thread's 16 entries of inv(L_bb) (row lr, columns
+    // 16q..16q+15) and the gathered right-hand side of its row
     const T* iv = inv + ioff[it.x] + int64_t(rb) * SB * SB + lr * SB + 16 * q;
     double ivr[16];
     for (int j = 0; j < 16; ++j) ivr[j] = double(iv[j]);
+    const double vr = (q == 0 && lr < nbp) ? v[r] : 0.0;
     double acc = 0.0;
     for (int cb = 0; cb < rb; ++cb) {
         double lv[16];
@@ -562,7 +564,7 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     }
     red[q][lr] = acc;
     __syncthreads();
-    if (q == 0) rs[lr] = lr < nbp ? v[r] - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]) : 0.0;
+    if (q == 0) rs[lr] = lr < nbp ? vr - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]) : 0.0;
     __syncthreads();
     // y = inv(L_bb) rs : thread (row lr, quarter q) sums 16 columns
     double part = 0.0;
@@ -645,6 +647,12 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
     }
 }
 
+// Backward chain link for pivot column block cb of a front. Everything that does not depend on
+// the next block's result is done before waiting for it: the partial sums of L21^T x_U and the
+// gathered y_b, the blocks cb2 > cb+1 (coalesced reads, lane = row, reduced over lanes once),
+// and the loads of the 64x64 block L[cb+1 rows][cb columns] and of this block's inverse; after
+// the flag of block cb+1 only 16 products per thread, two LDS reductions and the 64x64 GEMV
+// with the inverse remain on the critical path.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, int* __restrict__ flags,
@@ -669,13 +677,30 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     double* v = W + Fr.woff;
     int* fl = flags + foff[it.x];
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
-    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15), loaded before any wait
+    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15)
     const T* iv = inv + ioff[it.x] + int64_t(cb) * SB * SB + lane;
     double ivc[16];
     for (int j = 0; j < 16; ++j) ivc[j] = double(iv[(16 * wave + j) * SB]);
+    // L[64(cb+1) + 16w + j][c0 + lane]: the next block's rows against this thread's column
+    const bool has_next = cb + 1 < np;
+    const int nb1 = has_next ? min(64, p - 64 * (cb + 1)) : 0;
+    double ln[16];
+    for (int j = 0; j < 16; ++j) {
+        const int rr = 16 * wave + j;
+        ln[j] = (lane < nbc && rr < nb1) ? double(Lf[int64_t(c0 + lane) * m + 64 * (cb + 1) + rr]) : 0.0;
+    }
+    // y_b - L21^T x_U for this thread's column (wave 0)
+    double base = 0.0;
+    if (wave == 0 && lane < nbc) {
+        const double* pp = part + poff[it.x] + c0 + lane;
+        double t = 0.0;
+        for (int k = 0; k < nchunk; ++k) t += pp[int64_t(k) * p];
+        base = v[c0 + lane] - t;
+    }
+    // blocks cb2 = np-1 .. cb+2
     double acc[16];
     for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-    for (int cb2 = np - 1; cb2 > cb; --cb2) {
+    for (int cb2 = np - 1; cb2 > cb + 1; --cb2) {
         const int nb2 = min(64, p - 64 * cb2);
         double lv[16];
         for (int j = 0; j < 16; ++j) {
@@ -689,28 +714,32 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
         for (int j = 0; j < 16; ++j) acc[j] += lv[j] * xv;
     }
     for (int j = 0; j < 16; ++j) {
-        double s = acc[j];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (lane == 0) sred[wave + 4 * j] = s;
+        double sv = acc[j];
+        for (int o = 32; o > 0; o >>= 1) sv += __shfl_down(sv, o, 64);
+        if (lane == 0) sred[wave + 4 * j] = sv;
     }
+    // block cb+1
+    if (has_next && threadIdx.x == 0) wait_flag(fl + cb + 1, err);
     __syncthreads();
-    if (threadIdx.x < 64) {
-        double t = 0.0;
-        if (lane < nbc) {
-            const double* pp = part + poff[it.x] + c0 + lane;
-            for (int k = 0; k < nchunk; ++k) t += pp[int64_t(k) * p];
-            rs[lane] = v[c0 + lane] - t - sred[lane];
-        } else {
-            rs[lane] = 0.0;
+    double pn = 0.0;
+    if (has_next)
+        for (int j = 0; j < 16; ++j) {
+            const int rr = 16 * wave + j;
+            if (rr < nb1)
+                pn += ln[j] *
+                      __hip_atomic_load(xb + 64 * (cb + 1) + rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-    }
+    red[wave][lane] = pn;
+    __syncthreads();
+    if (wave == 0) rs[lane] = lane < nbc ? base - sred[lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]) : 0.0;
     __syncthreads();
     // x_b = inv(L_bb)^T rs : thread (column lane, quarter) sums 16 rows of inv
     double pt = 0.0;
     for (int j = 0; j < 16; ++j) pt += ivc[j] * rs[16 * wave + j];
+    __syncthreads();
     red[wave][lane] = pt;
     __syncthreads();
-    if (threadIdx.x < 64 && lane < nbc) {
+    if (wave == 0 && lane < nbc) {
         const double xv = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
         __hip_atomic_store(xb + 64 * cb + lane, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         v[c0 + lane] = xv;
