@@ -15,6 +15,7 @@
 #include "symbolic.hpp"
 
 #include <algorithm>
+#include <map>
 #include <climits>
 #include <cmath>
 #include <numeric>
@@ -362,7 +363,77 @@ Symbolic analyze(int n, const int* ig, const int* jg, const std::vector<char>& a
         std::vector<int> fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
         for (int f = 0; f < nf; ++f) S.level_fronts[fill[S.level[f]]++] = f;
     }
+    place_update_matrices(S);
     return S;
+}
+
+// U_f is written when front f is factored (its level) and read by the extend-add into its
+// parent, which finishes at the end of the parent's level; outside [level(f), level(parent)]
+// its storage is free. Offsets are assigned level by level, first fit over a free list of
+// intervals (largest matrices of a level first, 64-element alignment), which reduces the U
+// pool from the sum of all update matrices to the peak of the live ones.
+void place_update_matrices(Symbolic& S) {
+    const int nf = int(S.pcount.size());
+    std::map<int64_t, int64_t> free_list;  // offset -> length
+    int64_t top = 0;
+    std::vector<std::vector<int>> dies(S.nlevels + 1);
+    auto release = [&](int64_t off, int64_t len) {
+        auto it = free_list.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_list.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_list.erase(nx);
+        }
+        if (it != free_list.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_list.erase(it);
+            }
+        }
+    };
+    auto size_of = [&](int f) {
+        const int64_t u = S.mcount[f] - S.pcount[f];
+        return (u * u + 63) / 64 * 64;
+    };
+    for (int l = 0; l < S.nlevels; ++l) {
+        for (int f : dies[l]) release(S.uoff[f], size_of(f));
+        std::vector<int> fl(S.level_fronts.begin() + S.level_ptr[l], S.level_fronts.begin() + S.level_ptr[l + 1]);
+        std::stable_sort(fl.begin(), fl.end(), [&](int a, int b) { return size_of(a) > size_of(b); });
+        for (int f : fl) {
+            const int64_t need = size_of(f);
+            if (need == 0 || S.parent[f] < 0) {
+                S.uoff[f] = 0;
+                continue;
+            }
+            int64_t off = -1;
+            for (auto it = free_list.begin(); it != free_list.end(); ++it)
+                if (it->second >= need) {
+                    off = it->first;
+                    const int64_t rest = it->second - need;
+                    free_list.erase(it);
+                    if (rest > 0) free_list.emplace(off + need, rest);
+                    break;
+                }
+            if (off < 0) {
+                // extend the pool; a free interval ending at the top is merged into the new block
+                if (!free_list.empty()) {
+                    auto last = std::prev(free_list.end());
+                    if (last->first + last->second == top) {
+                        off = last->first;
+                        free_list.erase(last);
+                    }
+                }
+                if (off < 0) off = top;
+                top = off + need;
+            }
+            S.uoff[f] = off;
+            // freed after the parent's level (the extend-add into the parent runs at that level)
+            const int dl = S.level[S.parent[f]] + 1;
+            if (dl <= S.nlevels) dies[dl].push_back(f);
+        }
+    }
+    S.usize = top;
 }
 
 void assembly_map(const Symbolic& S, int n, const int* ig, const int* jg,

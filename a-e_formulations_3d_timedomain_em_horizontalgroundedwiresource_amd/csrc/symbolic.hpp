@@ -49,6 +49,10 @@ struct OrderingOptions {
 Symbolic analyze(int n, const int* ig, const int* jg, const std::vector<char>& active,
                  const std::vector<GridCoord>& coords, const OrderingOptions& opt);
 
+// Assigns U_f offsets by lifetime (level of f .. level of its parent) so that update matrices
+// that are never live at the same time share storage; sets S.uoff and S.usize.
+void place_update_matrices(Symbolic& S);
+
 // For each lower RSF entry (and the diagonal) of the original pattern, the position of the
 // entry in the front storage L (or -1 when it belongs to a singleton row/column).
 void assembly_map(const Symbolic& S, int n, const int* ig, const int* jg,
