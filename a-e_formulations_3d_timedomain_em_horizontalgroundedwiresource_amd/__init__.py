@@ -60,12 +60,30 @@ class CalcInfo(C.Structure):
                 ("last_residual", C.c_double)]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("rc", C.c_int), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("admit_s", C.c_double), ("end_s", C.c_double),
+                ("footprint_bytes", C.c_double), ("n_edges", C.c_int), ("steps", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("running", C.c_int64), ("waiting", C.c_int64), ("peak_running", C.c_int),
+                ("used_bytes", C.c_double), ("peak_bytes", C.c_double), ("budget_bytes", C.c_double),
+                ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
+                ("max_latency_s", C.c_double), ("throughput_rps", C.c_double)]
+
+
+REQ_QUEUED, REQ_ANALYSING, REQ_WAITING_MEMORY, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(6)
+
 EXPORTS = [
     "emtd_last_error", "emtd_portrait", "emtd_assemble",
     "emtd_solver_create", "emtd_solver_factorize", "emtd_solver_solve_nrhs", "emtd_solver_stop",
     "emtd_solver_destroy", "emtd_solver_get_stats",
     "emtd_calc_create", "emtd_calc_run", "emtd_calc_step", "emtd_calc_set_timing", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
     "emtd_calc_last_field", "emtd_calc_get_info", "emtd_calc_destroy", "emtd_calc_nonstat_3d_line",
+    "emtd_server_create", "emtd_server_submit", "emtd_server_wait", "emtd_server_poll", "emtd_server_drain",
+    "emtd_server_get_stats", "emtd_server_destroy",
 ]
 
 
@@ -79,6 +97,7 @@ def lib():
         L.emtd_last_error.restype = C.c_char_p
         L.emtd_solver_destroy.restype = None
         L.emtd_calc_destroy.restype = None
+        L.emtd_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -228,3 +247,50 @@ def calc_nonstat_3d_line(workdir: str, form: int = -1, scheme: int = -1, precisi
     """Run the module on ``workdir`` exactly as ``CalcNonstat3DLine.exe`` would."""
     _check(lib().emtd_calc_nonstat_3d_line(workdir.encode(), C.c_int(form), C.c_int(scheme),
                                            C.c_int(precision), C.c_int(device)))
+
+
+class Server:
+    """Request loop: forward-model requests (calculation directories) run concurrently on one
+    GPU, admitted by device-memory footprint in submission order (see include/emtd.h)."""
+
+    def __init__(self, device: int = -1, hbm_budget_bytes: float = 0.0, max_active: int = 4):
+        self.h = C.c_void_p()
+        _check(lib().emtd_server_create(C.c_int(device), C.c_double(hbm_budget_bytes), C.c_int(max_active),
+                                        C.byref(self.h)))
+
+    def submit(self, workdir: str, form: int = -1, scheme: int = -1, precision: int = FP64) -> int:
+        rid = C.c_int64(0)
+        _check(lib().emtd_server_submit(self.h, workdir.encode(), C.c_int(form), C.c_int(scheme),
+                                        C.c_int(precision), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, check: bool = True) -> RequestInfo:
+        inf = RequestInfo()
+        rc = lib().emtd_server_wait(self.h, C.c_int64(rid), C.byref(inf))
+        if check:
+            _check(rc)
+        return inf
+
+    def poll(self, rid: int) -> RequestInfo:
+        inf = RequestInfo()
+        _check(lib().emtd_server_poll(self.h, C.c_int64(rid), C.byref(inf)))
+        return inf
+
+    def drain(self):
+        _check(lib().emtd_server_drain(self.h))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().emtd_server_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().emtd_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

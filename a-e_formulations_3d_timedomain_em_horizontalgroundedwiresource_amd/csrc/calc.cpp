@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -30,20 +31,32 @@ using namespace emtd;
 
 namespace emtd {
 
-CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, int device)
+CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, int device,
+                   const std::function<void(int64_t)>& admit)
     : dir_(dir), device_(device), precision_(precision) {
     const double t0 = wall_seconds();
     select_device(device);
-    EMTD_HIP(hipStreamCreate(&stream_));
     M_ = read_model_files(dir, form);
     form_ = form >= 0 ? form : (M_.variant_form >= 0 ? M_.variant_form : 0);
     scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
     if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
     n_ = M_.n;
     npls_ = M_.npls;
-    // pattern and GPU assembly of B and C (original numbering)
+    // host: pattern, ordering and symbolic analysis (Dirichlet edges are singletons)
     build_portrait(M_.kpar, M_.ed.data(), n_, ig_, jg_);
     nnz_ = ig_[n_];
+    bnd_ = boundary_edge_mask(M_);
+    std::vector<char> active(n_);
+    for (int i = 0; i < n_; ++i) active[i] = !bnd_[i];
+    std::vector<GridCoord> coords = edge_grid_coords(M_.kuzlov, M_.xyz.data(), n_, M_.edges.data());
+    OrderingOptions opt;
+    S_ = std::make_unique<Symbolic>(analyze(n_, ig_.data(), jg_.data(), active, coords, opt));
+    nn_ = S_->nn;
+    // device memory is claimed only after the caller admitted the footprint
+    footprint_ = estimate_footprint();
+    if (admit) admit(footprint_);
+    EMTD_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // GPU assembly of B and C (original numbering)
     emtd_mesh mesh{M_.kuzlov, M_.kpar, n_, M_.nmat, M_.xyz.data(), M_.nver.data(), M_.ed.data(),
                    M_.edges.data(), M_.nvkat.data(), M_.sigma.data(), M_.mu.data()};
     b_ggl_.alloc(std::max<int64_t>(nnz_, 1));
@@ -55,14 +68,6 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         A.run(b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p);
         EMTD_HIP(hipStreamSynchronize(stream_));
     }
-    // ordering and symbolic analysis; Dirichlet edges are singletons
-    bnd_ = boundary_edge_mask(M_);
-    std::vector<char> active(n_);
-    for (int i = 0; i < n_; ++i) active[i] = !bnd_[i];
-    std::vector<GridCoord> coords = edge_grid_coords(M_.kuzlov, M_.xyz.data(), n_, M_.edges.data());
-    OrderingOptions opt;
-    S_ = std::make_unique<Symbolic>(analyze(n_, ig_.data(), jg_.data(), active, coords, opt));
-    nn_ = S_->nn;
     {
         std::vector<int64_t> doff, ddiag;
         assembly_map(*S_, n_, ig_.data(), jg_.data(), doff, ddiag);
@@ -187,6 +192,26 @@ CalcCore::~CalcCore() {
         if (stream_) (void)hipStreamDestroy(stream_);
     } catch (...) {
     }
+}
+
+// Device bytes this model will hold at its peak (factor storage of the chosen precision plus,
+// for FP32+IR, the FP64 fallback factor and the residual operator; matrices, maps, vectors).
+int64_t CalcCore::estimate_footprint() const {
+    const Symbolic& S = *S_;
+    const int64_t n = n_, nnz = nnz_, np = npls_, nn = nn_;
+    auto factor = [&](int64_t elem) { return (S.lsize + S.usize + nn * 64) * elem + S.wsize * 8 * np + nn * 160; };
+    int64_t b = 2 * (nnz + n) * 8;        // B and C values
+    b += (nnz + n) * 8;                   // scatter maps into the factor
+    b += (2 * nnz + n) * 12 + n * 8;      // C in permuted CSR
+    b += n * 4;                           // permutation
+    b += 12 * n * np * 8;                 // fields, RHS, work vectors, kept layers
+    if (precision_ == EMTD_FP64) {
+        b += factor(8);
+    } else {
+        b += factor(4) + factor(8);                   // FP32 factor and the FP64 fallback
+        b += (2 * nnz + n) * 20 + nn * 16 + 2 * n * np * 8;  // residual operator, IR vectors
+    }
+    return b + (int64_t(64) << 20);
 }
 
 bool CalcCore::stop_requested() const { return file_exists(join_path(dir_, "stop")); }
@@ -572,7 +597,9 @@ void CalcCore::receiver_values(int kind, double* out) const {
     const int nt_out = last_ - first_ + 1;
     std::vector<double> all(size_t(nt_out) * npls_ * rec_rows_);
     if (!all.empty()) {
-        EMTD_HIP(hipMemcpy(all.data(), rec_vals_.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+        EMTD_HIP(hipMemcpyAsync(all.data(), rec_vals_.p, all.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                stream_));
+        EMTD_HIP(hipStreamSynchronize(stream_));
     }
     const int nrec = kind == 0 ? nrecB_ : nrecE_;
     const int row0 = kind == 0 ? 0 : 3 * nrecB_;
@@ -587,7 +614,8 @@ void CalcCore::receiver_values(int kind, double* out) const {
 void CalcCore::last_field(double* out) const {
     if (!ran_) throw Error(EMTD_ERR_NOTINIT, "run before reading the field");
     std::vector<double> v(size_t(n_) * npls_);
-    EMTD_HIP(hipMemcpy(v.data(), last_field_.p, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EMTD_HIP(hipMemcpyAsync(v.data(), last_field_.p, v.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    EMTD_HIP(hipStreamSynchronize(stream_));
     for (int ip = 0; ip < npls_; ++ip)
         for (int i = 0; i < n_; ++i) out[size_t(ip) * n_ + i] = v[size_t(ip) * n_ + S_->iperm[i]];
 }

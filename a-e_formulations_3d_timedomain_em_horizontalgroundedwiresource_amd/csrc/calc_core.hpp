@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -19,7 +20,10 @@ namespace emtd {
 
 class CalcCore {
   public:
-    CalcCore(const std::string& dir, int form, int scheme, int precision, int device);
+    // admit (optional) is called with the estimated device footprint after the host-side
+    // analysis and before any device allocation; it may block (admission control) or throw.
+    CalcCore(const std::string& dir, int form, int scheme, int precision, int device,
+             const std::function<void(int64_t)>& admit = {});
     ~CalcCore();
     void run();
     int advance(int nsteps);  // runs up to nsteps steps (all when negative); synchronizes
@@ -30,6 +34,9 @@ class CalcCore {
     void last_field(double* out) const;
     void write_outputs() const;
     void info(emtd_calc_info* inf) const;
+    int64_t footprint() const { return footprint_; }
+    int n_edges() const { return n_; }
+    int64_t steps_done() const { return steps_; }
 
   private:
     struct LoopState {
@@ -42,6 +49,7 @@ class CalcCore {
         int tn = 0, tn1 = 0, tn2 = 0;
         double *uj = nullptr, *uj1 = nullptr, *uj2 = nullptr;
     };
+    int64_t estimate_footprint() const;
     void begin();
     void begin_decade();
     void two_layer_step();
@@ -103,6 +111,7 @@ class CalcCore {
     DevBuf<double> kvb_, kvc_, kdb_, kdc_, norms_, ir_r_, ir_d_;
     std::vector<double> h_norms_;
     int64_t k_nnz_ = 0;
+    int64_t footprint_ = 0;
     double setup_seconds_ = 0, factor_seconds_ = 0, step_seconds_ = 0;
     int64_t steps_ = 0, n_factorizations_ = 0, refine_iterations_ = 0, fallbacks_ = 0;
 };

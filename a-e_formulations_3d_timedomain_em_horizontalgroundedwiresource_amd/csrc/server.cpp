@@ -1,0 +1,353 @@
+// Persistent request loop: forward-model requests (one calculation directory each) run
+// concurrently on one GPU, admitted by device-memory footprint.
+//
+// A request goes through: queued -> analysing (host: model files, pattern, ordering; its
+// footprint becomes known) -> waiting for memory -> running (on its own non-blocking HIP
+// stream) -> done / failed. `max_active` worker threads each drive one request at a time, so up
+// to that many requests have work on the device at once and their kernels overlap; a new
+// request starts as soon as a worker and enough of the memory budget are free (continuous
+// admission, no batch boundaries). Memory is granted strictly in submission order, so a large
+// request is never starved by later small ones. Each request's results are the reference's
+// output files written into its own directory, exactly as the drop-in executable writes them;
+// the arithmetic of a request does not depend on what else runs (no cross-request state), so
+// its results are the same alone and in the mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/emtd.h"
+#include "calc_core.hpp"
+#include "common.hpp"
+
+using namespace emtd;
+
+namespace emtd {
+void set_last_error(const std::string& s);
+
+namespace {
+
+struct Request {
+    int64_t id = 0;
+    std::string dir;
+    int form = -1, scheme = -1, precision = EMTD_FP64;
+    int status = EMTD_REQ_QUEUED, rc = EMTD_OK;
+    std::string error;
+    double t_submit = 0, t_start = 0, t_admit = 0, t_end = 0;
+    int64_t footprint = 0;
+    int n_edges = 0;
+    int64_t steps = 0;
+    bool granted = false;
+};
+
+class Server {
+  public:
+    Server(int device, int64_t budget, int max_active) : device_(device), budget_(budget) {
+        t0_ = wall_seconds();
+        for (int k = 0; k < max_active; ++k) workers_.emplace_back([this] { worker(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stopping_ = true;
+        }
+        cv_work_.notify_all();
+        cv_mem_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    int64_t submit(const std::string& dir, int form, int scheme, int precision) {
+        std::lock_guard<std::mutex> g(mu_);
+        Request r;
+        r.id = next_id_++;
+        r.dir = dir;
+        r.form = form;
+        r.scheme = scheme;
+        r.precision = precision;
+        r.t_submit = now();
+        reqs_[r.id] = r;
+        queue_.push_back(r.id);
+        cv_work_.notify_one();
+        return r.id;
+    }
+    bool known(int64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        return reqs_.count(id) != 0;
+    }
+    Request wait(int64_t id) {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_done_.wait(g, [&] { return finished(reqs_.at(id)); });
+        return reqs_.at(id);
+    }
+    Request get(int64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        return reqs_.at(id);
+    }
+    void drain() {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_done_.wait(g, [&] {
+            for (auto& kv : reqs_)
+                if (!finished(kv.second)) return false;
+            return true;
+        });
+    }
+    void stats(emtd_server_stats* s) {
+        std::lock_guard<std::mutex> g(mu_);
+        *s = emtd_server_stats{};
+        std::vector<double> lat;
+        double first = 1e300, last = 0;
+        for (auto& kv : reqs_) {
+            const Request& r = kv.second;
+            ++s->submitted;
+            if (r.status == EMTD_REQ_DONE) {
+                ++s->completed;
+                lat.push_back(r.t_end - r.t_submit);
+                first = std::min(first, r.t_submit);
+                last = std::max(last, r.t_end);
+            } else if (r.status == EMTD_REQ_FAILED) {
+                ++s->failed;
+            } else if (r.status == EMTD_REQ_RUNNING) {
+                ++s->running;
+            } else {
+                ++s->waiting;
+            }
+        }
+        s->used_bytes = double(used_);
+        s->peak_bytes = double(peak_);
+        s->budget_bytes = double(budget_);
+        s->peak_running = peak_running_;
+        if (!lat.empty()) {
+            std::sort(lat.begin(), lat.end());
+            auto pct = [&](double q) {
+                // nearest-rank percentile
+                const size_t k = size_t(std::ceil(q * double(lat.size())));
+                return lat[std::min(lat.size() - 1, k > 0 ? k - 1 : 0)];
+            };
+            s->p50_latency_s = pct(0.50);
+            s->p99_latency_s = pct(0.99);
+            s->max_latency_s = lat.back();
+            double sum = 0;
+            for (double v : lat) sum += v;
+            s->mean_latency_s = sum / double(lat.size());
+            s->throughput_rps = last > first ? double(lat.size()) / (last - first) : 0.0;
+        }
+    }
+
+  private:
+    static bool finished(const Request& r) { return r.status == EMTD_REQ_DONE || r.status == EMTD_REQ_FAILED; }
+    double now() const { return wall_seconds() - t0_; }
+
+    // Blocks until `bytes` fit in the budget and every earlier request asking for memory has
+    // been served (grants in submission order).
+    void admit(int64_t id, int64_t bytes) {
+        std::unique_lock<std::mutex> g(mu_);
+        Request& r = reqs_.at(id);
+        r.footprint = bytes;
+        if (bytes > budget_) {
+            throw Error(EMTD_ERR_NOMEM, "request needs " + std::to_string(bytes) + " bytes of device memory, budget " +
+                                            std::to_string(budget_));
+        }
+        r.status = EMTD_REQ_WAITING_MEMORY;
+        waiting_.insert(id);
+        cv_mem_.wait(g, [&] { return stopping_ || (*waiting_.begin() == id && used_ + bytes <= budget_); });
+        waiting_.erase(id);
+        if (stopping_) throw Error(EMTD_ERR_STOPPED, "server stopping");
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        r.granted = true;
+        r.status = EMTD_REQ_RUNNING;
+        r.t_admit = now();
+        ++running_;
+        peak_running_ = std::max(peak_running_, running_);
+        cv_mem_.notify_all();
+    }
+    void release(int64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        Request& r = reqs_.at(id);
+        if (r.granted) {
+            used_ -= r.footprint;
+            r.granted = false;
+            --running_;
+        }
+        cv_mem_.notify_all();
+    }
+
+    void worker() {
+        for (;;) {
+            int64_t id;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_work_.wait(g, [&] { return stopping_ || !queue_.empty(); });
+                if (stopping_) return;
+                id = queue_.front();
+                queue_.pop_front();
+                Request& r = reqs_.at(id);
+                r.status = EMTD_REQ_ANALYSING;
+                r.t_start = now();
+            }
+            Request job = get(id);
+            int rc = EMTD_OK;
+            std::string err;
+            int n_edges = 0;
+            int64_t steps = 0;
+            try {
+                auto core = std::make_unique<CalcCore>(job.dir, job.form, job.scheme, job.precision, device_,
+                                                       [&](int64_t bytes) { admit(id, bytes); });
+                core->run();
+                core->write_outputs();
+                n_edges = core->n_edges();
+                steps = core->steps_done();
+            } catch (const Error& e) {
+                rc = e.code;
+                err = e.what();
+            } catch (const std::bad_alloc&) {
+                rc = EMTD_ERR_NOMEM;
+                err = "host allocation failed";
+            } catch (const std::exception& e) {
+                rc = EMTD_ERR_ERROR;
+                err = e.what();
+            }
+            release(id);  // the CalcCore (and its device memory) is gone here
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                Request& r = reqs_.at(id);
+                r.rc = rc;
+                r.error = err;
+                r.n_edges = n_edges;
+                r.steps = steps;
+                r.t_end = now();
+                r.status = rc == EMTD_OK ? EMTD_REQ_DONE : EMTD_REQ_FAILED;
+            }
+            cv_done_.notify_all();
+        }
+    }
+
+    int device_;
+    int64_t budget_;
+    double t0_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_mem_, cv_done_;
+    std::deque<int64_t> queue_;
+    std::map<int64_t, Request> reqs_;
+    std::set<int64_t> waiting_;
+    int64_t next_id_ = 1, used_ = 0, peak_ = 0;
+    int running_ = 0, peak_running_ = 0;
+    bool stopping_ = false;
+    std::vector<std::thread> workers_;
+};
+
+void fill_info(const Request& r, emtd_request_info* o) {
+    o->id = r.id;
+    o->status = r.status;
+    o->rc = r.rc;
+    o->submit_s = r.t_submit;
+    o->start_s = r.t_start;
+    o->admit_s = r.t_admit;
+    o->end_s = r.t_end;
+    o->footprint_bytes = double(r.footprint);
+    o->n_edges = r.n_edges;
+    o->steps = r.steps;
+}
+
+}  // namespace
+}  // namespace emtd
+
+struct emtd_server {
+    std::unique_ptr<Server> s;
+};
+
+#define SRV_GUARD_BEGIN try {
+#define SRV_GUARD_END                             \
+    }                                             \
+    catch (const Error& e) {                      \
+        set_last_error(e.what());                 \
+        return e.code;                            \
+    }                                             \
+    catch (const std::bad_alloc&) {               \
+        set_last_error("host allocation failed"); \
+        return EMTD_ERR_NOMEM;                    \
+    }                                             \
+    catch (const std::exception& e) {            \
+        set_last_error(e.what());                 \
+        return EMTD_ERR_ERROR;                    \
+    }
+
+extern "C" int emtd_server_create(int device, double hbm_budget_bytes, int max_active, emtd_server** out) {
+    SRV_GUARD_BEGIN
+    if (!out) throw Error(EMTD_ERR_RANGE, "null argument");
+    if (max_active < 1 || max_active > 64) throw Error(EMTD_ERR_RANGE, "max_active must be in 1..64");
+    select_device(device);
+    int64_t budget = int64_t(hbm_budget_bytes);
+    if (budget <= 0) {
+        size_t free_b = 0, total_b = 0;
+        EMTD_HIP(hipMemGetInfo(&free_b, &total_b));
+        budget = int64_t(double(free_b) * 0.9);
+    }
+    auto s = std::make_unique<emtd_server>();
+    s->s = std::make_unique<Server>(device, budget, max_active);
+    *out = s.release();
+    return EMTD_OK;
+    SRV_GUARD_END
+}
+
+extern "C" int emtd_server_submit(emtd_server* s, const char* workdir, int formulation, int scheme, int precision,
+                                  int64_t* id) {
+    SRV_GUARD_BEGIN
+    if (!s || !workdir || !id) throw Error(EMTD_ERR_RANGE, "null argument");
+    if (precision != EMTD_FP64 && precision != EMTD_FP32_IR) throw Error(EMTD_ERR_RANGE, "unknown precision");
+    *id = s->s->submit(workdir, formulation, scheme, precision);
+    return EMTD_OK;
+    SRV_GUARD_END
+}
+
+extern "C" int emtd_server_wait(emtd_server* s, int64_t id, emtd_request_info* info) {
+    SRV_GUARD_BEGIN
+    if (!s) throw Error(EMTD_ERR_RANGE, "null argument");
+    if (!s->s->known(id)) throw Error(EMTD_ERR_RANGE, "unknown request id");
+    Request r = s->s->wait(id);
+    if (info) fill_info(r, info);
+    if (r.rc != EMTD_OK) set_last_error(r.error);
+    return r.rc;
+    SRV_GUARD_END
+}
+
+extern "C" int emtd_server_poll(emtd_server* s, int64_t id, emtd_request_info* info) {
+    SRV_GUARD_BEGIN
+    if (!s || !info) throw Error(EMTD_ERR_RANGE, "null argument");
+    if (!s->s->known(id)) throw Error(EMTD_ERR_RANGE, "unknown request id");
+    Request r = s->s->get(id);
+    fill_info(r, info);
+    if (r.status == EMTD_REQ_FAILED) set_last_error(r.error);
+    return EMTD_OK;
+    SRV_GUARD_END
+}
+
+extern "C" int emtd_server_drain(emtd_server* s) {
+    SRV_GUARD_BEGIN
+    if (!s) throw Error(EMTD_ERR_RANGE, "null argument");
+    s->s->drain();
+    return EMTD_OK;
+    SRV_GUARD_END
+}
+
+extern "C" int emtd_server_get_stats(emtd_server* s, emtd_server_stats* st) {
+    SRV_GUARD_BEGIN
+    if (!s || !st) throw Error(EMTD_ERR_RANGE, "null argument");
+    s->s->stats(st);
+    return EMTD_OK;
+    SRV_GUARD_END
+}
+
+extern "C" void emtd_server_destroy(emtd_server* s) {
+    try {
+        delete s;
+    } catch (...) {
+    }
+}

@@ -392,3 +392,47 @@ def config_spec(cfg: int, small: tuple | None = None, seed: int | None = None) -
         tq = quantize_times(spec.times)
         spec.print_window = (tq[2], tq[-1])
     return spec
+
+
+def request_spec(seed: int, index: int, big: bool = False, axis_range: tuple = (24, 64),
+                 steps: tuple = (15, 10)) -> ModelSpec:
+    """One forward-model request of the batched / serving workloads.
+
+    Mesh cells per axis drawn uniformly from ``axis_range`` (about 40k-790k edges; ``big``:
+    100x100x66 cells, about 2M edges); a 2-layer earth (tops 0 and 50-300 m, conductivities
+    log-uniform in [1e-3, 1] S/m) plus 1-4 random blocks (0.001-2 S/m); A or E formulation
+    and 2- or 3-point scheme drawn at random; ``steps[0] * steps[1]`` steps to 10 ms with dt
+    doubling every ``steps[0]`` steps. The wire (x-directed, step-off) spans 80% and the
+    7-receiver profile 120% of the smaller horizontal half-extent. Seeded by (seed, index).
+    """
+    rng = np.random.default_rng([seed, index])
+    if big:
+        cells = (100, 100, 66)
+    else:
+        cells = tuple(int(v) for v in rng.integers(axis_range[0], axis_range[1] + 1, size=3))
+    cx, cy, cz = cells
+    h0, q = 10.0, 1.15
+    xs = graded_axis(cx, h0, q)
+    ys = graded_axis(cy, h0, q)
+    zs = graded_axis(cz, h0, q, nneg=cz // 2)
+    d1 = float(rng.uniform(50.0, 300.0))
+    sig = (10 ** rng.uniform(-3, 0, size=2)).tolist()
+    ext = min(xs[-1], ys[-1])
+    depth = -zs[0]
+    blocks = []
+    for _ in range(int(rng.integers(1, 5))):
+        w = float(rng.uniform(0.1, 0.3)) * ext
+        cxb, cyb = rng.uniform(-0.5 * ext, 0.5 * ext, size=2)
+        top = float(rng.uniform(0.05, 0.3)) * depth
+        th = float(rng.uniform(0.05, 0.2)) * depth
+        blocks.append(Block(cxb - w / 2, cxb + w / 2, cyb - w / 2, cyb + w / 2, -top - th, -top,
+                            float(10 ** rng.uniform(-3, np.log10(2.0)))))
+    form = "A" if rng.integers(0, 2) == 0 else "E"
+    scheme = int(rng.integers(2, 4))
+    spec = ModelSpec(xs, ys, zs, layer_tops=[0.0, d1], layer_sigma=sig, blocks=blocks,
+                     formulation=form, scheme=scheme, times=time_grid(1e-2, steps[0], steps[1]))
+    spec.wire = ((-0.4 * ext, 0.0, 0.0), (0.4 * ext, 0.0, 0.0))
+    spec.receivers = default_receivers(y=0.3 * ext, n=7, half=0.6 * ext)
+    tq = quantize_times(spec.times)
+    spec.print_window = (tq[2] if scheme == 3 else tq[1], tq[-1])
+    return spec

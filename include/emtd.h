@@ -150,6 +150,58 @@ void emtd_calc_destroy(emtd_calc* c);
 int emtd_calc_nonstat_3d_line(const char* workdir, int formulation, int scheme, int precision,
                               int device);
 
+/* ---------------------------------------------------------------- request loop
+ * No counterpart in the reference (which runs one model per process). For callers that serve
+ * many forward-model requests on one GPU: each request is a calculation directory with the
+ * reference's input files; results are written into that directory as emtd_calc_write_outputs
+ * writes them. Up to max_active requests run at once on their own streams; a request is
+ * admitted when its device footprint (factor + update pool + matrices + time levels, known
+ * after the host-side analysis) fits the remaining budget, in submission order. */
+#define EMTD_REQ_QUEUED 0
+#define EMTD_REQ_ANALYSING 1
+#define EMTD_REQ_WAITING_MEMORY 2
+#define EMTD_REQ_RUNNING 3
+#define EMTD_REQ_DONE 4
+#define EMTD_REQ_FAILED 5
+
+typedef struct emtd_server emtd_server;
+
+typedef struct emtd_request_info {
+    int64_t id;
+    int status;              /* EMTD_REQ_* */
+    int rc;                  /* EMTD_OK or the request's error code */
+    double submit_s;         /* times in seconds since the server was created */
+    double start_s;          /* a worker took the request */
+    double admit_s;          /* device memory granted */
+    double end_s;            /* finished (done or failed) */
+    double footprint_bytes;  /* estimated device footprint */
+    int n_edges;
+    int64_t steps;
+} emtd_request_info;
+
+typedef struct emtd_server_stats {
+    int64_t submitted, completed, failed, running, waiting;
+    int peak_running;        /* most requests holding device memory at once */
+    double used_bytes, peak_bytes, budget_bytes;
+    double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s; /* submit -> end, completed */
+    double throughput_rps;   /* completed / (last end - first submit of completed requests) */
+} emtd_server_stats;
+
+/* hbm_budget_bytes <= 0: 90% of the device's free memory at creation. */
+int emtd_server_create(int device, double hbm_budget_bytes, int max_active, emtd_server** out);
+/* formulation/scheme -1: from the directory's emtd_variant file (as emtd_calc_create). */
+int emtd_server_submit(emtd_server* s, const char* workdir, int formulation, int scheme, int precision,
+                       int64_t* id);
+/* Blocks until the request finished; returns its rc (message in emtd_last_error). */
+int emtd_server_wait(emtd_server* s, int64_t id, emtd_request_info* info);
+/* Non-blocking: fills info with the current status. */
+int emtd_server_poll(emtd_server* s, int64_t id, emtd_request_info* info);
+/* Blocks until every submitted request finished. */
+int emtd_server_drain(emtd_server* s);
+int emtd_server_get_stats(emtd_server* s, emtd_server_stats* stats);
+/* Requests not yet started are dropped; running ones finish first. */
+void emtd_server_destroy(emtd_server* s);
+
 #ifdef __cplusplus
 }
 #endif
