@@ -75,6 +75,54 @@ def cpu_baseline(cells=(20, 20, 16), steps_per_decade=20):
                        f"assembly + 1 factorization + {nsteps} steps + receivers, {dt:.1f} s")}
 
 
+def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), steps=(15, 10), max_active=6,
+                arrivals=True):
+    """Forward-model requests from the batched/serving generator through the request loop.
+
+    arrivals=True: Poisson arrivals at ``rate`` requests/s (seeded); False: all submitted at
+    once (batched workload). Models are generated before the clock starts. Returns throughput,
+    completion latency percentiles (submit -> tables written) and memory figures."""
+    import random
+    emtd = importlib.import_module(PKG)
+    mg = importlib.import_module(PKG + ".modelgen")
+    rng = random.Random(seed)
+    base = tempfile.mkdtemp(prefix="emtd_srv_")
+    try:
+        dirs, edges = [], 0
+        for i in range(n_requests):
+            big = rng.random() < big_fraction
+            spec = mg.request_spec(seed, i, big=big, axis_range=axis_range, steps=steps)
+            d = os.path.join(base, f"r{i}")
+            os.makedirs(d)
+            info = mg.write_model(spec, d)
+            edges += info["n"]
+            dirs.append(d)
+        gaps = [rng.expovariate(rate) if arrivals else 0.0 for _ in dirs]
+        srv = emtd.Server(max_active=max_active)
+        t0 = time.perf_counter()
+        ids = []
+        for d, g in zip(dirs, gaps):
+            if g > 0:
+                time.sleep(g)
+            ids.append(srv.submit(d))
+        srv.drain()
+        wall = time.perf_counter() - t0
+        infos = [srv.poll(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+        steps_done = sum(i.steps for i in infos)
+        return {"requests": n_requests, "completed": int(st.completed), "failed": int(st.failed),
+                "arrival_rate_rps": rate if arrivals else None, "big_fraction": big_fraction,
+                "total_edges": edges, "wall_s": round(wall, 3),
+                "throughput_rps": round(st.completed / wall, 4), "timesteps_per_s": round(steps_done / wall, 2),
+                "p50_latency_s": round(st.p50_latency_s, 4), "p99_latency_s": round(st.p99_latency_s, 4),
+                "mean_latency_s": round(st.mean_latency_s, 4), "max_latency_s": round(st.max_latency_s, 4),
+                "max_active": max_active, "peak_running": int(st.peak_running),
+                "peak_device_bytes": st.peak_bytes, "budget_bytes": st.budget_bytes}
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+
+
 def measured_traffic(args, cells):
     """HBM bytes per solve measured with rocprofv3 FETCH_SIZE/WRITE_SIZE on this workload (the
     summary under profiles/ gives the commands and the copy-kernel calibration), or None when
@@ -101,6 +149,9 @@ def main():
     ap.add_argument("--cells", type=str, default=None, help="override cells, e.g. 40x40x30")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the short serving sample")
+    ap.add_argument("--max-active", type=int, default=6, help="concurrent requests (configs 4/5)")
+    ap.add_argument("--requests", type=int, default=40, help="requests of the config-5 run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (3 times), for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -119,6 +170,27 @@ def main():
               f"the multi-GPU design is described in DESIGN.md and not measured.", file=sys.stderr)
         sys.exit(2)
     emtd = importlib.import_module(PKG)
+    if args.config in (4, 5):
+        # 4: 64 models of uneven sizes submitted at once; 5: Poisson arrivals at 4/s with 10%
+        # 2M-edge models. value = time steps completed per second over all requests.
+        if args.config == 4:
+            log("[bench] batched workload: 64 requests submitted at once")
+            r = serving_run(64, 0.0, seed=4, arrivals=False, max_active=args.max_active)
+        else:
+            log(f"[bench] serving workload: {args.requests} requests, Poisson 4/s, 10% 2M-edge models")
+            r = serving_run(args.requests, 4.0, seed=5, big_fraction=0.1, max_active=args.max_active)
+        log(f"[bench] {r}")
+        out = {"metric": METRIC, "value": r["timesteps_per_s"], "unit": "timesteps/s", "n_gpus": 1,
+               "steps": None, "warmup": 0, "ms_per_step": round(1e3 / max(r["timesteps_per_s"], 1e-9), 4),
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic",
+               "config": {"workload": ("config 4: 64 models, 24-64 cells per axis, mixed A/E and 2-/3-point, "
+                                       "150 steps each, submitted at once" if args.config == 4 else
+                                       "config 5: Poisson arrivals 4/s from the config-4 generator plus 10% "
+                                       "100x100x66-cell (2M edges) models, 150 steps each")},
+               "serving": r}
+        print(json.dumps(out), flush=True)
+        return
     cells = tuple(int(v) for v in args.cells.split("x")) if args.cells else None
     log(f"[bench] generating config {args.config} model {cells or ''}")
     t0 = time.time()
@@ -181,6 +253,12 @@ def main():
         c.close()
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    if not args.no_serving and args.config == 3 and cells is None:
+        # completion latency under load: 16 requests of the serving generator (24-48 cells per
+        # axis, 150 steps each) arriving as a Poisson stream at 4/s
+        log("[bench] serving sample")
+        out["serving"] = serving_run(16, 4.0, seed=5, axis_range=(24, 48), max_active=args.max_active)
+        log(f"[bench] serving: {out['serving']}")
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
