@@ -111,6 +111,11 @@ def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), s
         st = srv.stats()
         srv.close()
         steps_done = sum(i.steps for i in infos)
+        done = [i for i in infos if i.status == emtd.REQ_DONE]
+        mean = (lambda v: round(sum(v) / len(v), 4) if v else None)
+        phases = {"mean_queue_s": mean([i.start_s - i.submit_s for i in done]),
+                  "mean_analysis_s": mean([i.admit_s - i.start_s for i in done]),
+                  "mean_device_s": mean([i.end_s - i.admit_s for i in done])}
         return {"requests": n_requests, "completed": int(st.completed), "failed": int(st.failed),
                 "arrival_rate_rps": rate if arrivals else None, "big_fraction": big_fraction,
                 "total_edges": edges, "wall_s": round(wall, 3),
@@ -118,7 +123,7 @@ def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), s
                 "p50_latency_s": round(st.p50_latency_s, 4), "p99_latency_s": round(st.p99_latency_s, 4),
                 "mean_latency_s": round(st.mean_latency_s, 4), "max_latency_s": round(st.max_latency_s, 4),
                 "max_active": max_active, "peak_running": int(st.peak_running),
-                "peak_device_bytes": st.peak_bytes, "budget_bytes": st.budget_bytes}
+                "peak_device_bytes": st.peak_bytes, "budget_bytes": st.budget_bytes, **phases}
     finally:
         shutil.rmtree(base, ignore_errors=True)
 
