@@ -438,25 +438,66 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
 // Levels with few, large fronts split every front over many workgroups: 64-row blocks in
 // the forward sweep, 64-column pivot blocks in the backward sweep. A workgroup takes a ticket
 // (atomic counter) and processes items in ticket order, so it only ever waits for blocks held
-// by workgroups that started earlier; results of a pivot block are published through a
-// per-block buffer and flag (release/acquire at agent scope). The wait is bounded: after
-// kSpinLimit polls the workgroup records an error and continues, so the kernel always ends.
+// by workgroups that started earlier; the results of a pivot block are published through a
+// per-block buffer of slots (see kUnset). Every wait is bounded: after kSpinLimit polls the
+// workgroup records an error and continues, so the kernel always ends.
 constexpr int kSpinLimit = 1 << 22;
 
-__device__ __forceinline__ void wait_flag(const int* flag, int* err) {
-    int spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
+// Chain links hand results over through a buffer pre-filled with a signalling-NaN bit pattern:
+// a value is published by storing it (it is written once per solve), and a reader polls the
+// value itself, so no separate flag round trip is needed. Arithmetic never produces this
+// pattern (computed NaNs are quiet), so a NaN result cannot be mistaken for "not yet written".
+constexpr unsigned long long kUnset = 0xFFF4DEADBEEF0BADull;
+
+__device__ __forceinline__ unsigned long long load_slot(const double* p) {
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_slot(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls 16 slots (the same addresses for the whole wavefront) until all are written. Every
+// wait is bounded: after kSpinLimit polls the error word is set and the kernel goes on, so it
+// always ends (the solve then reports the error).
+__device__ __forceinline__ void poll16(const double* p, double* out, int* err) {
+    for (int spins = 0;; ++spins) {
+        bool all = true;
+        for (int j = 0; j < 16; ++j) {
+            const unsigned long long u = load_slot(p + j);
+            out[j] = __longlong_as_double((long long)u);
+            all = all && (u != kUnset);
         }
+        if (all) return;
+        if (spins > kSpinLimit) {
+            atomicOr(err, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
-__device__ __forceinline__ void publish_flag(int* flag) {
-    __threadfence();
-    __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+// Polls one slot per lane until every active lane's slot is written (inactive lanes get 0).
+__device__ __forceinline__ double poll1(const double* p, bool active, int* err) {
+    unsigned long long u = active ? load_slot(p) : 0ull;
+    for (int spins = 0; __any(u == kUnset); ++spins) {
+        if (spins > kSpinLimit) {
+            if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (u == kUnset) u = load_slot(p);
+    }
+    return active ? __longlong_as_double((long long)u) : 0.0;
+}
+
+// Before a chain launch: every slot unset and the ticket counter at zero.
+__global__ void k_chain_reset(unsigned long long* __restrict__ slots, int64_t n, int* __restrict__ counter) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) slots[i] = kUnset;
+    if (i == 0) *counter = 0;
 }
 
 // v_f = [b_P ; 0] + children's update vectors (one workgroup per front)
@@ -526,8 +567,7 @@ __global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, 
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
-                                                   const int* __restrict__ foff, int* __restrict__ flags,
-                                                   double* __restrict__ ybuf, int* __restrict__ counter,
+                                                   const int* __restrict__ foff, double* __restrict__ ybuf, int* __restrict__ counter,
                                                    int* __restrict__ err, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const int64_t* __restrict__ ioff,
                                                    double* __restrict__ W, double* __restrict__ x) {
@@ -544,7 +584,6 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     const int r = r0 + lr;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
-    int* fl = flags + foff[it.x];
     double* yb = ybuf + int64_t(foff[it.x]) * 64;
     // loaded before any wait: this thread's 16 entries of inv(L_bb) (row lr, columns
     // 16q..16q+15) and the gathered right-hand side of its row
@@ -554,13 +593,11 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     const double vr = (q == 0 && lr < nbp) ? v[r] : 0.0;
     double acc = 0.0;
     for (int cb = 0; cb < rb; ++cb) {
-        double lv[16];
+        double lv[16], yv[16];
         const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
         for (int j = 0; j < 16; ++j) lv[j] = lr < nbp ? double(lc[int64_t(j) * m]) : 0.0;
-        if (threadIdx.x == 0) wait_flag(fl + cb, err);
-        __syncthreads();
-        for (int j = 0; j < 16; ++j)
-            acc += lv[j] * __hip_atomic_load(yb + 64 * cb + 16 * q + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        poll16(yb + 64 * cb + 16 * q, yv, err);
+        for (int j = 0; j < 16; ++j) acc += lv[j] * yv[j];
     }
     red[q][lr] = acc;
     __syncthreads();
@@ -571,14 +608,15 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     for (int j = 0; j < 16; ++j) part += ivr[j] * rs[16 * q + j];
     red[q][lr] = part;
     __syncthreads();
-    if (q == 0 && lr < nbp) {
-        const double y = red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr];
-        __hip_atomic_store(yb + 64 * rb + lr, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v[r] = y;
-        x[Fr.fbeg + r] = y;
+    if (q == 0) {
+        // rows past the block end publish 0 so that readers of whole 16-slot groups finish
+        const double y = lr < nbp ? red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr] : 0.0;
+        if (lr < nbp) {
+            v[r] = y;
+            x[Fr.fbeg + r] = y;
+        }
+        store_slot(yb + 64 * rb + lr, y);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) publish_flag(fl + rb);
 }
 
 // w = v_U - L21 y for one 64-row block of update rows (y final: previous kernel).
@@ -655,8 +693,7 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
 // with the inverse remain on the critical path.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
-                                                   const int* __restrict__ foff, int* __restrict__ flags,
-                                                   double* __restrict__ xbuf, int* __restrict__ counter,
+                                                   const int* __restrict__ foff, double* __restrict__ xbuf, int* __restrict__ counter,
                                                    int* __restrict__ err, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const int64_t* __restrict__ ioff,
                                                    const double* __restrict__ part, const int64_t* __restrict__ poff,
@@ -675,7 +712,6 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
-    int* fl = flags + foff[it.x];
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
     // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15)
     const T* iv = inv + ioff[it.x] + int64_t(cb) * SB * SB + lane;
@@ -707,10 +743,7 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
             const int c = wave + 4 * j;
             lv[j] = (lane < nb2 && c < nbc) ? double(Lf[int64_t(c0 + c) * m + 64 * cb2 + lane]) : 0.0;
         }
-        if (threadIdx.x == 0) wait_flag(fl + cb2, err);
-        __syncthreads();
-        const double xv =
-            lane < nb2 ? __hip_atomic_load(xb + 64 * cb2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        const double xv = poll1(xb + 64 * cb2 + lane, lane < nb2, err);
         for (int j = 0; j < 16; ++j) acc[j] += lv[j] * xv;
     }
     for (int j = 0; j < 16; ++j) {
@@ -718,17 +751,13 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
         for (int o = 32; o > 0; o >>= 1) sv += __shfl_down(sv, o, 64);
         if (lane == 0) sred[wave + 4 * j] = sv;
     }
-    // block cb+1
-    if (has_next && threadIdx.x == 0) wait_flag(fl + cb + 1, err);
-    __syncthreads();
+    // block cb+1 (its rows past the block end are published as 0)
     double pn = 0.0;
-    if (has_next)
-        for (int j = 0; j < 16; ++j) {
-            const int rr = 16 * wave + j;
-            if (rr < nb1)
-                pn += ln[j] *
-                      __hip_atomic_load(xb + 64 * (cb + 1) + rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    if (has_next) {
+        double xn[16];
+        poll16(xb + 64 * (cb + 1) + 16 * wave, xn, err);
+        for (int j = 0; j < 16; ++j) pn += ln[j] * xn[j];
+    }
     red[wave][lane] = pn;
     __syncthreads();
     if (wave == 0) rs[lane] = lane < nbc ? base - sred[lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]) : 0.0;
@@ -739,14 +768,14 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     __syncthreads();
     red[wave][lane] = pt;
     __syncthreads();
-    if (wave == 0 && lane < nbc) {
-        const double xv = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
-        __hip_atomic_store(xb + 64 * cb + lane, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v[c0 + lane] = xv;
-        x[Fr.fbeg + c0 + lane] = xv;
+    if (wave == 0) {
+        const double xv = lane < nbc ? red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane] : 0.0;
+        if (lane < nbc) {
+            v[c0 + lane] = xv;
+            x[Fr.fbeg + c0 + lane] = xv;
+        }
+        store_slot(xb + 64 * cb + lane, xv);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) publish_flag(fl + cb);
 }
 
 inline unsigned grid_for(int64_t n, int block) {
@@ -1014,7 +1043,7 @@ void GpuCholesky<T>::prepare_big_levels() {
     foff_ = to_device(foff, stream_);
     poff_ = to_device(poff, stream_);
     max_flags_ = maxflags;
-    flags_.alloc(size_t(max_flags_) + 2);  // flags + ticket counter + error word
+    flags_.alloc(size_t(max_flags_) + 2);  // ticket counter at [max_flags_], error word after it
     xbuf_.alloc(size_t(max_flags_) * 64);
     part_.alloc(size_t(maxpart));
     std::vector<int> z(max_flags_ + 2, 0);
@@ -1075,8 +1104,9 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
     int* counter = flags_.p + max_flags_;
     int* err = flags_.p + max_flags_ + 1;
     auto clear_flags = [&]() {
-        hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(max_flags_ + 1, 256)), dim3(256), 0, stream_, flags_.p,
-                           int64_t(max_flags_ + 1), 0);
+        const int64_t ns = int64_t(max_flags_) * 64;
+        hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256)), dim3(256), 0, stream_,
+                           reinterpret_cast<unsigned long long*>(xbuf_.p), ns, counter);
     };
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
@@ -1092,7 +1122,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
             hipLaunchKernelGGL(k_fwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, children_.p, relmap_.p,
                                Wr, d_b + r * ld);
             hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               fc_items_.p + lv_fc_[l], foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
                                ioff_.p, Wr, d_x + r * ld);
             if (lv_l21_[l + 1] > lv_l21_[l])
                 hipLaunchKernelGGL(k_fwd_l21<T>, dim3(lv_l21_[l + 1] - lv_l21_[l]), dim3(256), 0, stream_, fronts_.p,
@@ -1116,7 +1146,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                    fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
             clear_flags();
             hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               bc_items_.p + lv_bc_[l], foff_.p, flags_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
                                ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);
         }
     }
