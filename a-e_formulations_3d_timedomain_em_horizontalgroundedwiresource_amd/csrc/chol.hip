@@ -669,11 +669,25 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
     const double* v = W + Fr.woff;
     double acc[16];
     for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-    for (int r = rb + lane; r < re; r += 64) {
-        const double xv = v[r];
+    // two 64-row slices per iteration: 32 independent loads in flight per thread
+    int r = rb + lane;
+    for (; r + 64 < re; r += 128) {
+        const double xa = v[r], xb = v[r + 64];
+        double la[16], lb[16];
         for (int j = 0; j < 16; ++j) {
             const int c = wave + 4 * j;
-            if (c < nbc) acc[j] += double(Lf[int64_t(c0 + c) * m + r]) * xv;
+            const T* col = Lf + int64_t(c0 + c) * m + r;
+            la[j] = c < nbc ? double(col[0]) : 0.0;
+            lb[j] = c < nbc ? double(col[64]) : 0.0;
+        }
+        for (int j = 0; j < 16; ++j) acc[j] += la[j] * xa;
+        for (int j = 0; j < 16; ++j) acc[j] += lb[j] * xb;
+    }
+    if (r < re) {
+        const double xa = v[r];
+        for (int j = 0; j < 16; ++j) {
+            const int c = wave + 4 * j;
+            if (c < nbc) acc[j] += double(Lf[int64_t(c0 + c) * m + r]) * xa;
         }
     }
     double* out = part + poff[it.x] + int64_t(it.z) * p + c0;
