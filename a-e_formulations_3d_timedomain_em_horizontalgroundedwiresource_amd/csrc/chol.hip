@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
     const int r0 = kb + nb + 64 * blockIdx.x;
     if (blockIdx.x > 0 && r0 >= Fr.m) return;
     const int nwg = max(1, (Fr.m - kb - nb + 63) / 64);
-    __shared__ T colj[NB];
+    __shared__ T colj[2][NB];
     __shared__ int s_last;
     const int t = threadIdx.x;
     const int64_t m = Fr.m;
@@ -141,28 +141,29 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
         a[j] = (diag && j < nb && t >= j) ? col[j * m + kb + t] : T(0);
         x[j] = (row && j < nb) ? col[j * m + r] : T(0);
     }
+    // Right-looking: the pivot comes from lane j by a shuffle; every lane forms the same square
+    // root and reciprocal, scales its entry of column j and publishes it in LDS (two buffers
+    // alternate, so one barrier per pivot suffices); then every lane updates its own row.
     int bad = -1;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (j < nb) {
-            if (t == j) colj[j] = a[j];
-            __syncthreads();
-            T d = colj[j];
+            T d = __shfl(a[j], j, 64);
             if (!(d > T(0))) {
                 if (bad < 0) bad = j;
                 d = T(1);
             }
             d = sqrt(d);
+            const T rd = T(1) / d;
             if (t == j) a[j] = d;
-            if (t > j) a[j] /= d;
+            if (t > j) a[j] *= rd;
+            if (diag && t >= j) colj[j & 1][t] = a[j];  // column j of the factor, rows j..nb-1
             __syncthreads();
-            if (diag && t >= j) colj[t] = a[j];  // column j of the factor, rows j..nb-1
-            __syncthreads();
-            x[j] /= d;
+            x[j] *= rd;
 #pragma unroll
             for (int l = j + 1; l < NB; ++l) {
                 if (l < nb) {
-                    const T c = colj[l];
+                    const T c = colj[j & 1][l];
                     a[l] -= a[j] * c;
                     x[l] -= x[j] * c;
                 }
@@ -210,8 +211,9 @@ struct Mfma<float> {
 
 // Symmetric update of the lower frontal matrix:
 //   F[r][c] -= sum_{k in [k0, k1)} L[r][k] L[c][k]   for r >= c, c in [cs, ce), r in [cs, m)
-// with cs = k1 and ce = min(cmax, p) for the pivot columns (inside the current outer block,
-// cmax = kb + NBO, and the trailing pivot columns, cmax = INT_MAX), and, for cmax < 0, the
+// with cs = max(k1, cmin) and ce = min(cmax, p) for the pivot columns (inside the current outer
+// block, and the trailing pivot columns in two parts: the next outer block and the rest), and,
+// for cmax < 0, the
 // Schur complement of the update block in one pass: U = -L21 L21^T (k0 = 0, k1 = p, cs = p,
 // ce = m), stored without reading U; the children's contributions to U are added afterwards.
 // TM x TM tiles (blockIdx.x = row tile, blockIdx.y = column tile, blockIdx.z = front) over
@@ -227,7 +229,7 @@ constexpr int UPAD = 4;
 
 template <typename T, int TM, int WV, int KC>
 __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                        int k0, int k1, int cmax, T* __restrict__ L,
+                                                        int k0, int k1, int cmin, int cmax, T* __restrict__ L,
                                                         T* __restrict__ U) {
     constexpr int NT = 64 * WV * WV, WT = TM / WV, NA = WT / 16, E = TM * KC / NT, KS = NT / TM;
     static_assert(NT % TM == 0 && (TM * KC) % NT == 0 && WT % 16 == 0, "tile shape");
@@ -237,7 +239,7 @@ __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restr
     const bool ublock = cmax < 0;
     const int kend = min(k1, p);
     if (kend <= k0) return;
-    const int cs = ublock ? p : kend;
+    const int cs = ublock ? p : max(kend, cmin);
     const int ce = ublock ? m : min(cmax, p);
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (ti < tj) return;
@@ -870,16 +872,27 @@ GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stre
 // over 16 waves for large blocks, 64x64 tiles over 4 waves for small ones; K chunks of 128
 // bytes per row (16 FP64 / 32 FP32 values).
 template <typename T>
-void GpuCholesky<T>::update(const int* fz, int cnt, int k0, int k1, int cmax, int rows, int cols) {
+GpuCholesky<T>::~GpuCholesky() {
+    if (side_) {
+        (void)hipStreamSynchronize(side_);
+        (void)hipStreamDestroy(side_);
+        (void)hipEventDestroy(ev_main_);
+        (void)hipEventDestroy(ev_side_);
+    }
+}
+
+template <typename T>
+void GpuCholesky<T>::update(hipStream_t st, const int* fz, int cnt, int k0, int k1, int cmin, int cmax, int rows,
+                            int cols) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
     if (cols >= 256) {
         const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
-        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), 0, stream_, fronts_.p, fz, k0, k1,
-                           cmax, L_.p, U_.p);
+        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), 0, st, fronts_.p, fz, k0, k1,
+                           cmin, cmax, L_.p, U_.p);
     } else {
         const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
-        hipLaunchKernelGGL((k_update<T, 64, 2, KC>), dim3(tr, tc, cnt), dim3(256), 0, stream_, fronts_.p, fz, k0, k1,
+        hipLaunchKernelGGL((k_update<T, 64, 2, KC>), dim3(tr, tc, cnt), dim3(256), 0, st, fronts_.p, fz, k0, k1, cmin,
                            cmax, L_.p, U_.p);
     }
 }
@@ -895,9 +908,26 @@ void GpuCholesky<T>::extend_add(int l, const std::vector<int>& ptr, const int2* 
     }
 }
 
+// Look-ahead: the trailing update after outer block kb is split into the next outer block's
+// columns (on the main stream, right away) and the rest (on a second stream), so the panel
+// chain of block kb+NBO runs while the bulk of the update is still in flight. The next trailing
+// update writes the same columns as the bulk, so it waits for it (event), and the main stream
+// waits for the second one before the update block and at the end of every level.
 template <typename T>
 void GpuCholesky<T>::numeric() {
     const Symbolic& S = S_;
+    if (!side_) {
+        EMTD_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        EMTD_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
+        EMTD_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
+    }
+    bool side_pending = false;
+    auto join_side = [&]() {
+        if (!side_pending) return;
+        EMTD_HIP(hipEventRecord(ev_side_, side_));
+        EMTD_HIP(hipStreamWaitEvent(stream_, ev_side_, 0));
+        side_pending = false;
+    };
     for (int l = 0; l < S.nlevels; ++l) {
         if (l > 0) extend_add(l, ea_ptr_, ea_items_.p);
         const std::vector<int>& fl = level_sorted_[l];
@@ -920,17 +950,29 @@ void GpuCholesky<T>::numeric() {
                     hipLaunchKernelGGL(k_panel<T>, dim3(std::max(1, (rem + 63) / 64), cnt), dim3(64), 0, stream_,
                                        fronts_.p, fz, ks, L_.p, info_.p, panel_done_.p);
                     // inside the outer block: columns [ks+NB, kb+NBO) with K = NB
-                    update(fz, cnt, ks, ks + NB, kb + NBO, rem, std::min(kb + NBO, mp) - (ks + 1));
+                    update(stream_, fz, cnt, ks, ks + NB, 0, kb + NBO, rem, std::min(kb + NBO, mp) - (ks + 1));
                 }
-                // trailing pivot columns [kb+NBO, p) with K = NBO, rows to m
-                update(fz, cnt, kb, kb + NBO, INT_MAX, mm - (kb + NBO), mp - (kb + NBO));
+                // the previous bulk update wrote the columns the trailing updates touch
+                join_side();
+                const int k1 = kb + NBO, k2 = kb + 2 * NBO;
+                // the remaining pivot columns [kb+2NBO, p) on the side stream, once the block's
+                // columns are final
+                if (mp > k2) {
+                    EMTD_HIP(hipEventRecord(ev_main_, stream_));
+                    EMTD_HIP(hipStreamWaitEvent(side_, ev_main_, 0));
+                    update(side_, fz, cnt, kb, k1, k2, INT_MAX, mm - k2, mp - k2);
+                    side_pending = true;
+                }
+                // next outer block's columns [kb+NBO, kb+2NBO), K = NBO, all rows
+                update(stream_, fz, cnt, kb, k1, 0, k2, mm - k1, std::min(k2, mp) - k1);
             }
         }
+        join_side();
         // update block of every front: U = -L21 L21^T with K = p, one pass
         int mu = 0;
         for (int f : fl) mu = std::max(mu, S.mcount[f] - S.pcount[f]);
         for (int nz = 0; nz < int(fl.size()); nz += 65535)
-            update(dfl + nz, std::min(65535, int(fl.size()) - nz), 0, INT_MAX, -1, mu, mu);
+            update(stream_, dfl + nz, std::min(65535, int(fl.size()) - nz), 0, INT_MAX, 0, -1, mu, mu);
         if (l > 0) extend_add(l, eau_ptr_, eau_items_.p);
     }
     EMTD_HIP(hipGetLastError());

@@ -27,6 +27,9 @@ template <typename T>
 class GpuCholesky {
   public:
     GpuCholesky(const Symbolic& S, hipStream_t stream);
+    ~GpuCholesky();
+    GpuCholesky(const GpuCholesky&) = delete;
+    GpuCholesky& operator=(const GpuCholesky&) = delete;
 
     // Numeric factorization of  mult*C + B  (+1e-12 diagonal bump, Dirichlet rows dropped) from
     // the RSF value arrays of B and C on the device (original numbering, lower pattern).
@@ -51,13 +54,15 @@ class GpuCholesky {
 
   private:
     void numeric();
-    void update(const int* fz, int cnt, int k0, int k1, int cmax, int rows, int cols);
+    void update(hipStream_t st, const int* fz, int cnt, int k0, int k1, int cmin, int cmax, int rows, int cols);
     void extend_add(int l, const std::vector<int>& ptr, const int2* items);
     void prepare_big_levels();
     void prepare_inverses();
     void invert_diagonal_blocks();
     const Symbolic& S_;
     hipStream_t stream_;
+    hipStream_t side_ = nullptr;  // second stream of the factorization (look-ahead)
+    hipEvent_t ev_main_ = nullptr, ev_side_ = nullptr;
     DevBuf<T> L_, U_;
     DevBuf<double> W_;
     DevBuf<DevFront> fronts_;

@@ -77,6 +77,7 @@ class ModelSpec:
     times: np.ndarray | None = None
     current: np.ndarray | None = None                    # I(t_i); step-off when None
     wire: tuple = ((-500.0, 0.0, 0.0), (500.0, 0.0, 0.0))
+    extra_wires: list = field(default_factory=list)      # more sources (npls = 1 + len)
     receivers: np.ndarray | None = None                  # (nrec, 3)
     n_neib: int = 3
     n_srs_razb: int = 500
@@ -295,11 +296,13 @@ def write_model(spec: ModelSpec, out: str) -> dict:
     with open(os.path.join(out, "currentfunction"), "w") as f:
         for ti, ci in zip(tq, cur):
             f.write(f"{ti:.7e} {ci:.17g}\n")
-    (ax, ay, az), (bx, by, bz) = spec.wire
+    wires = [spec.wire] + list(spec.extra_wires)
+    npls = len(wires)
     with open(os.path.join(out, "sours"), "w") as f:
-        f.write(f"{ax:.17g} {ay:.17g} {az:.17g}\n{bx:.17g} {by:.17g} {bz:.17g}\n")
+        for (ax, ay, az), (bx, by, bz) in wires:
+            f.write(f"{ax:.17g} {ay:.17g} {az:.17g}\n{bx:.17g} {by:.17g} {bz:.17g}\n")
     with open(os.path.join(out, "group"), "w") as f:
-        f.write("1\n1 1 1\n")
+        f.write(f"1\n1 1 {npls}\n")
     rec = spec.receivers if spec.receivers is not None else default_receivers()
     for name in ("xyzVectorB", "xyzVectorE"):
         with open(os.path.join(out, name), "w") as f:
@@ -308,7 +311,7 @@ def write_model(spec: ModelSpec, out: str) -> dict:
                 f.write(f"{r[0]:.17g} {r[1]:.17g} {r[2]:.17g}\n")
     for name in ("recvsb", "recvse"):
         with open(os.path.join(out, name), "w") as f:
-            f.write(f"{len(rec)}\n")
+            f.write("".join(f"{len(rec)}\n" for _ in range(npls)))
     left, right = spec.print_window if spec.print_window else (tq[1], tq[-1])
     with open(os.path.join(out, "timeintervalforprint"), "w") as f:
         f.write(f"{left:.17g} {right:.17g}\n")
@@ -319,15 +322,15 @@ def write_model(spec: ModelSpec, out: str) -> dict:
 
     hmin = float(min(np.diff(xs).min(), np.diff(ys).min(), np.diff(zs).min()))
     if spec.formulation == "A":
-        a0 = wire_vector_potential(xyz, spec.wire[0], spec.wire[1], rmin=0.5 * hmin)
+        a0 = np.concatenate([wire_vector_potential(xyz, w[0], w[1], rmin=0.5 * hmin).ravel() for w in wires])
         _w_f64(os.path.join(out, "a0.dat"), a0)
     else:
         s1 = spec.layer_sigma[0] if spec.layer_sigma else 0.01
-        v = electrode_potential(xyz, spec.wire[0], spec.wire[1], s1, rmin=0.5 * hmin)
+        v = np.concatenate([electrode_potential(xyz, w[0], w[1], s1, rmin=0.5 * hmin).ravel() for w in wires])
         _w_f64(os.path.join(out, "v3a.dat"), v)
     with open(os.path.join(out, "emtd_variant"), "w") as f:
         f.write(f"{spec.formulation} {spec.scheme}\n")
-    return dict(n=n, kuzlov=kuz, kpar=kpar, nt=len(t), nmat=nmat)
+    return dict(n=n, kuzlov=kuz, kpar=kpar, nt=len(t), nmat=nmat, npls=npls)
 
 
 def default_receivers(y: float = 300.0, n: int = 21, half: float = 1000.0) -> np.ndarray:

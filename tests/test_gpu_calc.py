@@ -121,3 +121,44 @@ def test_calc_fp32_refined_matches_fp64(emtd, tmp_path, form, scheme):
         print("kind", kind, "fp32-ir vs fp64 max rel err", err)
         assert err < TOL32
     assert inf.last_residual <= 1e-10 or inf.fallbacks > 0
+
+
+@pytest.mark.parametrize("form,scheme", [("A", 2), ("E", 3)])
+def test_calc_two_sources_matches_oracle(emtd, tmp_path, form, scheme):
+    """npls = 2 (two grounded wires): every source's receivers and the tables vs the oracle."""
+    from conftest import modelgen
+    mg = modelgen()
+    spec = mg.config_spec(1, small=(8, 8, 6))
+    spec.times = mg.time_grid(1e-3, 5, 3)
+    spec.formulation, spec.scheme = form, scheme
+    tq = mg.quantize_times(spec.times)
+    spec.print_window = (tq[2], tq[-1]) if scheme == 3 else (tq[1], tq[-1])
+    ext = min(spec.xs[-1], spec.ys[-1])
+    spec.extra_wires = [((0.0, -0.3 * ext, 0.0), (0.0, 0.3 * ext, 0.0))]
+    d = str(tmp_path)
+    mg.write_model(spec, d)
+    print("case two sources", form, scheme, flush=True)
+    m = refio.read_model(d)
+    assert m.npls == 2
+    ref = calc.run(m, form, scheme)
+    c = emtd.Calc(d, 0 if form == "A" else 1, scheme, emtd.FP64)
+    c.run()
+    for kind, want in ((0, ref.valB), (1, ref.valE)):
+        got = c.receiver_values(kind)
+        assert got.shape == want.shape
+        for ip in range(2):
+            scale = np.abs(want[:, ip]).max()
+            err = np.abs(got[:, ip] - want[:, ip]).max() / scale
+            print("kind", kind, "source", ip, "max rel err", err)
+            assert err < TOL64
+    c.write_outputs()
+    c.close()
+    od = os.path.join(d, "oracle_out")
+    os.makedirs(od)
+    refio.write_tables(od, m, ref.first, ref.last, ref.eds, ref.bfield, ref.efield, form)
+    for stem in ("eall_anom.1", "edsall_anom.1", "ball_anom.1"):
+        a = _read_table(os.path.join(d, stem))
+        b = _read_table(os.path.join(od, stem))
+        assert a.shape == b.shape, stem
+        scale = max(np.abs(b[:, 1:]).max(), 1e-300)
+        assert np.abs(a[:, 1:] - b[:, 1:]).max() / scale < TOL64, stem

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
 
-from conftest import make_model
+from conftest import make_model, modelgen
 from oracle import calc, fem, refio
 
 
@@ -219,3 +219,33 @@ def test_superlu_matches_dense(tmp_path):
     x = spla.splu(K).solve(b)
     xd = np.linalg.solve(K.toarray(), b)
     assert np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
+
+
+def test_sources_superpose(tmp_path):
+    """Two wires in one run (npls = 2): each source's receiver values equal those of a run with
+    that wire alone (the system is linear and the sources share nothing but the matrix)."""
+    mg = modelgen()
+    spec = mg.config_spec(1, small=(6, 6, 4))
+    spec.times = mg.time_grid(1e-3, 3, 2)
+    tq = mg.quantize_times(spec.times)
+    spec.print_window = (tq[1], tq[-1])
+    ext = min(spec.xs[-1], spec.ys[-1])
+    w0 = spec.wire
+    w1 = ((0.0, -0.3 * ext, 0.0), (0.0, 0.3 * ext, 0.0))
+    runs = {}
+    for name, wires in (("both", [w0, w1]), ("w0", [w0]), ("w1", [w1])):
+        spec.wire = wires[0]
+        spec.extra_wires = wires[1:]
+        d = tmp_path / name
+        d.mkdir()
+        mg.write_model(spec, str(d))
+        m = refio.read_model(str(d))
+        assert m.npls == len(wires)
+        runs[name] = calc.run(m, "A", 2)
+    for k, single in ((0, "w0"), (1, "w1")):
+        for attr in ("valB", "valE"):
+            a = getattr(runs["both"], attr)[:, k]
+            b = getattr(runs[single], attr)[:, 0]
+            scale = np.abs(b).max()
+            assert scale > 0
+            assert np.abs(a - b).max() / scale < 1e-10, (k, attr)
