@@ -109,6 +109,18 @@ __global__ void k_extend_add(const DevFront* __restrict__ F, const int2* __restr
     }
 }
 
+// 1/sqrt(d) for d > 0: the hardware estimate followed by two Newton steps
+// y <- y (3 - d y^2) / 2, each of which roughly doubles the number of correct digits.
+__device__ __forceinline__ double rsqrt_newton(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * (1.5 - 0.5 * d * y * y);
+    return y * (1.5 - 0.5 * d * y * y);
+}
+__device__ __forceinline__ float rsqrt_newton(float d) {
+    float y = __builtin_amdgcn_rsqf(d);
+    return y * (1.5f - 0.5f * d * y * y);
+}
+
 // One panel step of the blocked factorization, for the columns [kb, kb+nb) of each active
 // front. Every workgroup (one wavefront, 64 panel rows) factors the nb x nb diagonal block
 // itself (the same arithmetic in every workgroup, so all copies agree): lanes 0..nb-1 hold the
@@ -141,32 +153,37 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
         a[j] = (diag && j < nb && t >= j) ? col[j * m + kb + t] : T(0);
         x[j] = (row && j < nb) ? col[j * m + r] : T(0);
     }
-    // Right-looking: the pivot comes from lane j by a shuffle; every lane forms the same square
-    // root and reciprocal, scales its entry of column j and publishes it in LDS (two buffers
-    // alternate, so one barrier per pivot suffices); then every lane updates its own row.
+    // Right-looking. Every lane keeps the running diagonal dd[l] = A[l][l] - sum_k L[l][k]^2, so
+    // the pivot needs no communication; its reciprocal square root (hardware estimate refined
+    // by two Newton steps to full precision) scales column j, which is published in LDS (two
+    // buffers alternate, so one barrier per pivot suffices) and applied by every lane to its
+    // diagonal-block row, its panel row and its copy of the diagonal.
+    T dd[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) dd[j] = j < nb ? col[j * m + kb + j] : T(1);
     int bad = -1;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (j < nb) {
-            T d = __shfl(a[j], j, 64);
+            T d = dd[j];
             if (!(d > T(0))) {
                 if (bad < 0) bad = j;
                 d = T(1);
             }
-            d = sqrt(d);
-            const T rd = T(1) / d;
-            if (t == j) a[j] = d;
-            if (t > j) a[j] *= rd;
+            T y = rsqrt_newton(d);
+            if (t == j) a[j] = d * y;
+            if (t > j) a[j] *= y;
             if (diag && t >= j) colj[j & 1][t] = a[j];  // column j of the factor, rows j..nb-1
             __syncthreads();
-            x[j] *= rd;
+            x[j] *= y;
+            T cc[NB];
+#pragma unroll
+            for (int l = j + 1; l < NB; ++l) cc[l] = colj[j & 1][l];
 #pragma unroll
             for (int l = j + 1; l < NB; ++l) {
-                if (l < nb) {
-                    const T c = colj[j & 1][l];
-                    a[l] -= a[j] * c;
-                    x[l] -= x[j] * c;
-                }
+                dd[l] -= cc[l] * cc[l];
+                a[l] -= a[j] * cc[l];
+                x[l] -= x[j] * cc[l];
             }
         }
     }
