@@ -512,39 +512,37 @@ __device__ __forceinline__ double poll1(const double* p, bool active, int* err) 
     return active ? __longlong_as_double((long long)u) : 0.0;
 }
 
-// Before a chain launch: every slot unset and the ticket counter at zero.
-__global__ void k_chain_reset(unsigned long long* __restrict__ slots, int64_t n, int* __restrict__ counter) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) slots[i] = kUnset;
+// Forward gather of a split-sweep level, one thread per front row, and the reset of the chain
+// slots and ticket counter for the chain that follows:
+//   W[dst[i]] = (bsrc[i] >= 0 ? b[bsrc[i]] : 0) + sum of the children's update entries listed
+//   for that row (fixed order, so the sum is deterministic).
+__global__ void __launch_bounds__(256) k_fwd_gather_rows(int nrows, const int64_t* __restrict__ dst,
+                                                         const int64_t* __restrict__ bsrc,
+                                                         const int* __restrict__ cptr, const int64_t* __restrict__ cidx,
+                                                         double* __restrict__ W, const double* __restrict__ b,
+                                                         unsigned long long* __restrict__ slots, int nslots,
+                                                         int* __restrict__ counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslots) slots[i] = kUnset;
     if (i == 0) *counter = 0;
+    if (i >= nrows) return;
+    const int64_t s = bsrc[i];
+    double v = s >= 0 ? b[s] : 0.0;
+    for (int e = cptr[i]; e < cptr[i + 1]; ++e) v += W[cidx[e]];
+    W[dst[i]] = v;
 }
 
-// v_f = [b_P ; 0] + children's update vectors (one workgroup per front)
-__global__ void __launch_bounds__(256) k_fwd_gather(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                    const int* __restrict__ children, const int* __restrict__ relmap,
-                                                    double* __restrict__ W, const double* __restrict__ b) {
-    const DevFront Fr = F[flist[blockIdx.x]];
-    double* v = W + Fr.woff;
-    for (int i = threadIdx.x; i < Fr.m; i += blockDim.x) v[i] = i < Fr.p ? b[Fr.fbeg + i] : 0.0;
-    __syncthreads();
-    for (int ci = Fr.child_beg; ci < Fr.child_end; ++ci) {
-        const DevFront Fc = F[children[ci]];
-        const int uc = Fc.m - Fc.p;
-        const int* rm = relmap + Fc.relmap_off;
-        const double* wc = W + Fc.woff + Fc.p;
-        for (int k = threadIdx.x; k < uc; k += blockDim.x) v[rm[k]] += wc[k];
-        __syncthreads();
-    }
-}
-
-// v_f = [y_P ; x_U] for the backward sweep (one workgroup per front)
-__global__ void __launch_bounds__(256) k_bwd_gather(const DevFront* __restrict__ F, const int* __restrict__ flist,
-                                                    const int* __restrict__ rows, double* __restrict__ W,
-                                                    const double* __restrict__ x) {
-    const DevFront Fr = F[flist[blockIdx.x]];
-    double* v = W + Fr.woff;
-    const int* rw = rows + Fr.rows_off;
-    for (int i = threadIdx.x; i < Fr.m; i += blockDim.x) v[i] = i < Fr.p ? x[Fr.fbeg + i] : x[rw[i]];
+// Backward gather of a split-sweep level: W[dst[i]] = x[src[i]] (the front's y_P and x_U),
+// plus the reset of the chain slots and ticket counter.
+__global__ void __launch_bounds__(256) k_bwd_gather_rows(int nrows, const int64_t* __restrict__ dst,
+                                                         const int* __restrict__ src, double* __restrict__ W,
+                                                         const double* __restrict__ x,
+                                                         unsigned long long* __restrict__ slots, int nslots,
+                                                         int* __restrict__ counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslots) slots[i] = kUnset;
+    if (i == 0) *counter = 0;
+    if (i < nrows) W[dst[i]] = x[src[i]];
 }
 
 // ------------------------------------------------------------------ split sweeps
@@ -1070,6 +1068,9 @@ void GpuCholesky<T>::prepare_big_levels() {
     lv_bc_.assign(S.nlevels + 1, 0);
     int maxflags = 1;
     int64_t maxpart = 1;
+    std::vector<int64_t> gdst, gbsrc, gcidx;
+    std::vector<int> gxsrc, gcptr{0};
+    lv_rows_.assign(S.nlevels + 1, 0);
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         big_[l] = nfl < big_level_fronts_ ? 1 : 0;
@@ -1101,7 +1102,30 @@ void GpuCholesky<T>::prepare_big_levels() {
                 }
             maxflags = std::max(maxflags, off);
             maxpart = std::max(maxpart, po);
+            // gather rows of the level's fronts
+            for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
+                const int f = S.level_fronts[k];
+                const int p = S.pcount[f], m = S.mcount[f];
+                for (int i = 0; i < m; ++i) {
+                    gdst.push_back(S.woff[f] + i);
+                    gbsrc.push_back(i < p ? int64_t(S.fbeg[f] + i) : -1);
+                    gxsrc.push_back(i < p ? S.fbeg[f] + i : S.rows[S.rows_ptr[f] + i]);
+                }
+                // children's update entries, child by child
+                std::vector<std::vector<int64_t>> per(m);
+                for (int ci = S.child_ptr[f]; ci < S.child_ptr[f + 1]; ++ci) {
+                    const int c = S.children[ci];
+                    const int uc = S.mcount[c] - S.pcount[c];
+                    for (int kk = 0; kk < uc; ++kk)
+                        per[S.relmap[S.relmap_ptr[c] + kk]].push_back(S.woff[c] + S.pcount[c] + kk);
+                }
+                for (int i = 0; i < m; ++i) {
+                    for (int64_t e : per[i]) gcidx.push_back(e);
+                    gcptr.push_back(int(gcidx.size()));
+                }
+            }
         }
+        lv_rows_[l + 1] = int64_t(gdst.size());
         lv_fc_[l + 1] = int(fc.size());
         lv_l21_[l + 1] = int(l21.size());
         lv_l21t_[l + 1] = int(l21t.size());
@@ -1115,6 +1139,18 @@ void GpuCholesky<T>::prepare_big_levels() {
     l21t_items_ = to_device(l21t, stream_);
     foff_ = to_device(foff, stream_);
     poff_ = to_device(poff, stream_);
+    if (gdst.empty()) {
+        gdst.push_back(0);
+        gbsrc.push_back(-1);
+        gxsrc.push_back(0);
+        gcptr.push_back(0);
+    }
+    if (gcidx.empty()) gcidx.push_back(0);
+    g_dst_ = to_device(gdst, stream_);
+    g_bsrc_ = to_device(gbsrc, stream_);
+    g_xsrc_ = to_device(gxsrc, stream_);
+    g_cptr_ = to_device(gcptr, stream_);
+    g_cidx_ = to_device(gcidx, stream_);
     max_flags_ = maxflags;
     flags_.alloc(size_t(max_flags_) + 2);  // ticket counter at [max_flags_], error word after it
     xbuf_.alloc(size_t(max_flags_) * 64);
@@ -1176,11 +1212,8 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
     }
     int* counter = flags_.p + max_flags_;
     int* err = flags_.p + max_flags_ + 1;
-    auto clear_flags = [&]() {
-        const int64_t ns = int64_t(max_flags_) * 64;
-        hipLaunchKernelGGL(k_chain_reset, dim3(grid_for(ns, 256)), dim3(256), 0, stream_,
-                           reinterpret_cast<unsigned long long*>(xbuf_.p), ns, counter);
-    };
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(xbuf_.p);
+    const int nslots = max_flags_ * 64;
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         const int* lf = level_fronts_.p + S.level_ptr[l];
@@ -1191,9 +1224,10 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
         }
         for (int r = 0; r < nrhs; ++r) {
             double* Wr = W_.p + r * S.wsize;
-            clear_flags();
-            hipLaunchKernelGGL(k_fwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, children_.p, relmap_.p,
-                               Wr, d_b + r * ld);
+            const int nr = int(lv_rows_[l + 1] - lv_rows_[l]), r0 = int(lv_rows_[l]);
+            hipLaunchKernelGGL(k_fwd_gather_rows, dim3((std::max(nr, nslots) + 255) / 256), dim3(256), 0, stream_, nr,
+                               g_dst_.p + r0, g_bsrc_.p + r0, g_cptr_.p + r0, g_cidx_.p, Wr, d_b + r * ld, slots,
+                               nslots, counter);
             hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
                                fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
                                ioff_.p, Wr, d_x + r * ld);
@@ -1212,12 +1246,12 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
         }
         for (int r = 0; r < nrhs; ++r) {
             double* Wr = W_.p + r * S.wsize;
-            hipLaunchKernelGGL(k_bwd_gather, dim3(nfl), dim3(256), 0, stream_, fronts_.p, lf, rows_.p, Wr,
-                               d_x + r * ld);
+            const int nr = int(lv_rows_[l + 1] - lv_rows_[l]), r0 = int(lv_rows_[l]);
+            hipLaunchKernelGGL(k_bwd_gather_rows, dim3((std::max(nr, nslots) + 255) / 256), dim3(256), 0, stream_, nr,
+                               g_dst_.p + r0, g_xsrc_.p + r0, Wr, d_x + r * ld, slots, nslots, counter);
             if (lv_l21t_[l + 1] > lv_l21t_[l])
                 hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(lv_l21t_[l + 1] - lv_l21t_[l]), dim3(256), 0, stream_,
                                    fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
-            clear_flags();
             hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
                                bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
                                ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);

@@ -83,6 +83,10 @@ class GpuCholesky {
     DevBuf<int> foff_, flags_;
     DevBuf<int64_t> poff_, ioff_;
     DevBuf<double> xbuf_, part_;
+    // flattened gather rows of the split-sweep levels (per level ranges in lv_rows_)
+    std::vector<int64_t> lv_rows_;
+    DevBuf<int64_t> g_dst_, g_bsrc_, g_cidx_;
+    DevBuf<int> g_xsrc_, g_cptr_;
     DevBuf<T> inv_;
     int n_inv_items_ = 0;
     int max_flags_ = 1;
