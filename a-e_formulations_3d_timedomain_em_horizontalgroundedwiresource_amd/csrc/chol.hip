@@ -381,6 +381,7 @@ __global__ void __launch_bounds__(256) k_fwd(const DevFront* __restrict__ F, con
         __syncthreads();
     }
     __shared__ double y[SB];
+    __shared__ double rdg[SB];
     __shared__ double D[SB][SB + 1];
     const T* Lf = L + Fr.loff;
     const int lane = threadIdx.x & 63;
@@ -389,20 +390,33 @@ __global__ void __launch_bounds__(256) k_fwd(const DevFront* __restrict__ F, con
         stage_diag(D, Lf, m, kb, nb);
         __syncthreads();
         if (threadIdx.x < 64) {
+            rdg[lane] = lane < nb ? 1.0 / D[lane][lane] : 0.0;
             double val = lane < nb ? v[kb + lane] : 0.0;
             for (int j = 0; j < nb; ++j) {
-                const double yj = __shfl(val, j, 64) / D[j][j];
+                const double yj = __shfl(val, j, 64) * rdg[j];
                 if (lane == j) y[j] = yj;
                 if (lane > j && lane < nb) val -= D[lane][j] * yj;
             }
             if (lane < nb) v[kb + lane] = y[lane];
         }
         __syncthreads();
+        // rows below the block: 16 independent loads in flight per thread
         for (int r = kb + nb + threadIdx.x; r < m; r += blockDim.x) {
-            double acc = v[r];
             const T* lr = Lf + int64_t(kb) * m + r;
-            for (int j = 0; j < nb; ++j) acc -= double(lr[int64_t(j) * m]) * y[j];
-            v[r] = acc;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int j = 0;
+            for (; j + 16 <= nb; j += 16) {
+                double l[16];
+                for (int q = 0; q < 16; ++q) l[q] = double(lr[int64_t(j + q) * m]);
+                for (int q = 0; q < 16; q += 4) {
+                    a0 += l[q] * y[j + q];
+                    a1 += l[q + 1] * y[j + q + 1];
+                    a2 += l[q + 2] * y[j + q + 2];
+                    a3 += l[q + 3] * y[j + q + 3];
+                }
+            }
+            for (; j < nb; ++j) a0 += double(lr[int64_t(j) * m]) * y[j];
+            v[r] -= (a0 + a1) + (a2 + a3);
         }
         __syncthreads();
     }
@@ -424,25 +438,39 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
     for (int i = threadIdx.x; i < m; i += blockDim.x) v[i] = i < p ? xx[Fr.fbeg + i] : xx[rw[i]];
     __syncthreads();
     __shared__ double s[SB];
+    __shared__ double rdg[SB];
     __shared__ double D[SB][SB + 1];
     const T* Lf = L + Fr.loff;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nwaves = blockDim.x >> 6;
     for (int kb = ((p - 1) / SB) * SB; kb >= 0; kb -= SB) {
         const int nb = min(SB, p - kb);
-        for (int j = wave; j < nb; j += nwaves) {
-            const T* lc = Lf + int64_t(kb + j) * m;
-            double part = 0.0;
-            for (int r = kb + nb + lane; r < m; r += 64) part += double(lc[r]) * v[r];
-            part = wave_sum(part);
-            if (lane == 0) s[j] = part;
+        // s[c] = sum over the rows below the block of L[r][kb+c] v[r]: wave w owns the columns
+        // w, w+4, ..., lane = row, 16 independent loads in flight per thread
+        double acc[16];
+        for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+        for (int r = kb + nb + lane; r < m; r += 64) {
+            const double xv = v[r];
+            double l[16];
+            for (int q = 0; q < 16; ++q) {
+                const int c = wave + 4 * q;
+                l[q] = c < nb ? double(Lf[int64_t(kb + c) * m + r]) : 0.0;
+            }
+            for (int q = 0; q < 16; ++q) acc[q] += l[q] * xv;
+        }
+        for (int q = 0; q < 16; ++q) {
+            const int c = wave + 4 * q;
+            if (c < nb) {  // uniform in the wave
+                const double sum = wave_sum(acc[q]);
+                if (lane == 0) s[c] = sum;
+            }
         }
         stage_diag(D, Lf, m, kb, nb);
         __syncthreads();
         if (threadIdx.x < 64) {
+            rdg[lane] = lane < nb ? 1.0 / D[lane][lane] : 0.0;
             double val = lane < nb ? v[kb + lane] - s[lane] : 0.0;
             for (int j = nb - 1; j >= 0; --j) {
-                const double xj = __shfl(val, j, 64) / D[j][j];
+                const double xj = __shfl(val, j, 64) * rdg[j];
                 if (lane == j) s[j] = xj;
                 if (lane < j) val -= D[j][lane] * xj;
             }
