@@ -8,11 +8,12 @@ RHS (mass-matrix SpMV + source), forward/backward triangular sweeps with the GPU
 receiver evaluation; the first step of each decade also refactorizes the system matrix.
 
 Prints ONE JSON line. ``value`` = timed steps / wall time of the timed region (inputs and
-factor storage already resident in HBM). The roofline object covers the dominant per-step
-kernels (one triangular solve = the forward and backward sweep kernels): algorithmic bytes =
-8 bytes x nnz(L) read by each of the two sweeps, per right-hand side, divided by the solve's GPU
-time measured with HIP events on the solver's stream; ``traffic`` is the HBM bytes per solve
-measured with rocprofv3 counters on this workload (profiles/sweeps_config3_traffic.md).
+factor storage already resident in HBM). Two per-step hot paths are measured with HIP events
+on the solver's stream: one triangular solve (algorithmic bytes = 8 bytes x nnz(L) read by each
+of the two sweeps, per right-hand side; HBM roofline) and one refactorization (flops = sum over
+fronts of p^3/3 + p^2 u + p u^2; FP64 MFMA roofline). ``roofline`` is the one with the larger
+share of the timed region, ``roofline_other`` the other; ``traffic`` is the HBM bytes per launch
+measured with rocprofv3 counters on this workload (summaries under profiles/).
 """
 from __future__ import annotations
 
@@ -128,17 +129,18 @@ def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), s
         shutil.rmtree(base, ignore_errors=True)
 
 
-def measured_traffic(args, cells):
-    """HBM bytes per solve measured with rocprofv3 FETCH_SIZE/WRITE_SIZE on this workload (the
-    summary under profiles/ gives the commands and the copy-kernel calibration), or None when
-    the run is not the profiled workload."""
+def measured_traffic(args, cells, what):
+    """HBM bytes per solve ("solve") or per factorization ("factor") measured with rocprofv3
+    FETCH_SIZE/WRITE_SIZE on this workload (the summaries under profiles/ give the commands
+    and the copy-kernel calibration), or None when the run is not the profiled workload."""
     if args.config != 3 or cells is not None or args.precision != "fp64":
         return None
-    path = os.path.join(ROOT, "profiles", "sweeps_config3_traffic.md")
+    name = {"solve": "sweeps_config3_traffic.md", "factor": "factorization_config3_traffic.md"}[what]
+    key = {"solve": "traffic_bytes_per_solve:", "factor": "traffic_bytes_per_factorization:"}[what]
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_solve:"):
+                if line.startswith(key):
                     return int(line.split(":")[1])
     except OSError:
         pass
@@ -248,13 +250,29 @@ def main():
                        "timed_factorizations": nfac, "factor_ms": round(fac_gpu * 1e3, 3),
                        "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
                        "setup_s": round(ci.setup_seconds, 2)},
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": measured_traffic(args, cells),
-                         "kernel": ("one triangular solve (forward + backward sweep kernels) per step; "
-                                    "bytes = 2 sweeps x %d B x nnz(L) x sources" % elem),
-                         "bytes_per_launch": bytes_solve},
+            "roofline": None,
         }
+        # the two per-step hot paths; "roofline" is the one with the larger share of the timed
+        # region (the other is reported beside it)
+        peak_tf = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+        r_solve = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args, cells, "solve"),
+                   "kernel": ("one triangular solve (forward + backward sweep kernels); bytes = 2 sweeps x %d B "
+                              "x nnz(L) x sources" % elem),
+                   "bytes_per_launch": bytes_solve, "launch_ms": round(solve_s * 1e3, 4),
+                   "share_of_timed_region": round(nsolve * solve_s / elapsed, 4)}
+        r_fact = None
+        if nfac:
+            r_fact = {"bound": "mfma", "achieved": round(fac_tflops, 3), "peak": peak_tf, "unit": "TFLOP/s",
+                      "frac": round(fac_tflops / peak_tf, 4), "traffic": measured_traffic(args, cells, "factor"),
+                      "kernel": ("one multifrontal factorization (panel, update, extend-add kernels); flops = "
+                                 "sum over fronts of p^3/3 + p^2 u + p u^2"),
+                      "flops_per_launch": ci.factor_flops, "launch_ms": round(fac_gpu * 1e3, 3),
+                      "share_of_timed_region": round(nfac * fac_gpu / elapsed, 4)}
+        if r_fact and r_fact["share_of_timed_region"] > r_solve["share_of_timed_region"]:
+            out["roofline"], out["roofline_other"] = r_fact, r_solve
+        else:
+            out["roofline"], out["roofline_other"] = r_solve, r_fact
         c.close()
     finally:
         shutil.rmtree(d, ignore_errors=True)
