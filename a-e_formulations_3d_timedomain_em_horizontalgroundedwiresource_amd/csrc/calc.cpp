@@ -262,6 +262,13 @@ void CalcCore::factorize(double mult) {
     cur_mult_ = mult;
     int bad = -1;
     use64_ = (precision_ == EMTD_FP64);
+    // A smaller mult (a longer step) weakens the mass term that keeps the near-null space of
+    // the curl-curl operator (gradients in the air) positive definite, so once the FP32
+    // factorization has failed at some mult it is not tried again at a smaller or equal one.
+    if (!use64_ && fp32_failed_mult_ > 0 && mult <= fp32_failed_mult_) {
+        use64_ = true;
+        ++fallbacks_;
+    }
     if (!use64_) {
         c32_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
         bad = c32_->check_info();
@@ -269,6 +276,7 @@ void CalcCore::factorize(double mult) {
         if (bad >= 0) {  // FP32 pivot failure: this decade runs on an FP64 factor
             use64_ = true;
             ++fallbacks_;
+            fp32_failed_mult_ = std::max(fp32_failed_mult_, mult);
         }
     }
     if (use64_) {

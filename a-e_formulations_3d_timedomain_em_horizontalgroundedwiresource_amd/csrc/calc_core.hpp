@@ -105,6 +105,7 @@ class CalcCore {
     int64_t n_solves_ = 0, timed_solves_ = 0;
     bool use64_ = true;
     double cur_mult_ = 0, ir_tol_ = 1e-10, last_residual_ = 0;
+    double fp32_failed_mult_ = 0;  // largest mult at which the FP32 factorization failed
     int ir_max_ = 12;
     DevBuf<int64_t> krp_;
     DevBuf<int> kci_;

@@ -249,7 +249,9 @@ def main():
                        "precision": args.precision, "npls": ci.npls,
                        "timed_factorizations": nfac, "factor_ms": round(fac_gpu * 1e3, 3),
                        "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
-                       "setup_s": round(ci.setup_seconds, 2)},
+                       "setup_s": round(ci.setup_seconds, 2),
+                       "refine_iterations_per_step": round((inf1.refine_iterations - inf0.refine_iterations) / K, 4),
+                       "fallbacks": int(inf1.fallbacks), "final_relative_residual": inf1.last_residual},
             "roofline": None,
         }
         # the two per-step hot paths; "roofline" is the one with the larger share of the timed
