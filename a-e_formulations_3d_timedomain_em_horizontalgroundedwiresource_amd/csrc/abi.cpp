@@ -207,8 +207,8 @@ extern "C" int emtd_solver_solve_nrhs(emtd_solver* s, int nrhs, const double* pr
     EMTD_HIP(hipEventRecord(e1, s->stream));
     s->x.download(x, size_t(n) * nrhs, s->stream);
     EMTD_HIP(hipStreamSynchronize(s->stream));
-    if ((s->c64 ? s->c64->sweep_error() : s->c32->sweep_error()) != 0)
-        throw Error(EMTD_ERR_ERROR, "triangular sweep wait limit reached");
+    const std::string serr = s->c64 ? s->c64->sweep_error() : s->c32->sweep_error();
+    if (!serr.empty()) throw Error(EMTD_ERR_ERROR, serr);
     float ms = 0;
     EMTD_HIP(hipEventElapsedTime(&ms, e0, e1));
     s->st.solve_seconds = ms * 1e-3;

@@ -592,8 +592,9 @@ int CalcCore::advance(int nsteps) {
     int done = 0;
     while ((nsteps < 0 || done < nsteps) && step()) ++done;
     EMTD_HIP(hipStreamSynchronize(stream_));
-    if ((c64_ && c64_->sweep_error() != 0) || (c32_ && c32_->sweep_error() != 0))
-        throw Error(EMTD_ERR_ERROR, "triangular sweep wait limit reached");
+    for (const std::string& serr : {c64_ ? c64_->sweep_error() : std::string(),
+                                    c32_ ? c32_->sweep_error() : std::string()})
+        if (!serr.empty()) throw Error(EMTD_ERR_ERROR, serr);
     step_seconds_ += (wall_seconds() - t0) - (factor_seconds_ - f0);
     return done;
 }

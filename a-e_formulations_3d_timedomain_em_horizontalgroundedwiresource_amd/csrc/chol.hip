@@ -506,10 +506,48 @@ __device__ __forceinline__ void store_slot(double* p, double v) {
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Where a chain link waits: the record written by the first wait that gives up.
+struct WaitSite {
+    SweepDiag* dg;
+    const int* counter;
+    int kind, level, ticket;
+    int2 item;
+};
+
+__device__ __forceinline__ bool wait_abandoned(const SweepDiag* dg) {
+    return __hip_atomic_load(&dg->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// Called by one lane when a wait reaches kSpinLimit: the first such wait of the solve stores
+// where it stood and what the slot held (as the polling load saw it and as read by an atomic
+// read-modify-write at agent and at system scope), then the error word is set, which makes
+// every other wait of the solve give up at its next poll.
+__device__ void record_timeout(const WaitSite& ws, const double* slot, unsigned long long seen, int waited, int s,
+                               long long t0) {
+    SweepDiag* dg = ws.dg;
+    if (atomicCAS(&dg->claimed, 0, 1) == 0) {
+        dg->kind = ws.kind;
+        dg->level = ws.level;
+        dg->ticket = ws.ticket;
+        dg->front = ws.item.x;
+        dg->block = ws.item.y;
+        dg->waited_block = waited;
+        dg->slot = s;
+        dg->tickets_taken = __hip_atomic_load(ws.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dg->seen = seen;
+        unsigned long long* q = reinterpret_cast<unsigned long long*>(const_cast<double*>(slot));
+        dg->agent_rmw = __hip_atomic_fetch_or(q, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dg->system_rmw = __hip_atomic_fetch_or(q, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        dg->wait_ticks = wall_clock64() - t0;
+    }
+    atomicOr(&dg->err, 1);
+}
+
 // Polls 16 slots (the same addresses for the whole wavefront) until all are written. Every
-// wait is bounded: after kSpinLimit polls the error word is set and the kernel goes on, so it
-// always ends (the solve then reports the error).
-__device__ __forceinline__ void poll16(const double* p, double* out, int* err) {
+// wait is bounded: after kSpinLimit polls, or as soon as another wait of the solve has given
+// up, it returns (the solve then reports the error), so the kernel always ends.
+__device__ __forceinline__ void poll16(const double* p, double* out, const WaitSite& ws, int waited, int s0) {
+    const long long t0 = wall_clock64();
     for (int spins = 0;; ++spins) {
         bool all = true;
         for (int j = 0; j < 16; ++j) {
@@ -517,21 +555,30 @@ __device__ __forceinline__ void poll16(const double* p, double* out, int* err) {
             out[j] = __longlong_as_double((long long)u);
             all = all && (u != kUnset);
         }
-        if (all) return;
+        if (all || wait_abandoned(ws.dg)) return;
         if (spins > kSpinLimit) {
-            atomicOr(err, 1);
+            if ((threadIdx.x & 63) == 0) {
+                int j = 0;
+                while (j < 15 && (unsigned long long)__double_as_longlong(out[j]) != kUnset) ++j;
+                record_timeout(ws, p + j, (unsigned long long)__double_as_longlong(out[j]), waited, s0 + j, t0);
+            }
             return;
         }
         __builtin_amdgcn_s_sleep(1);
     }
 }
 
-// Polls one slot per lane until every active lane's slot is written (inactive lanes get 0).
-__device__ __forceinline__ double poll1(const double* p, bool active, int* err) {
+// Polls one slot per lane (slot = lane of block `waited`) until every active lane's slot is
+// written (inactive lanes get 0); bounded like poll16.
+__device__ __forceinline__ double poll1(const double* p, bool active, const WaitSite& ws, int waited) {
+    const long long t0 = wall_clock64();
     unsigned long long u = active ? load_slot(p) : 0ull;
     for (int spins = 0; __any(u == kUnset); ++spins) {
+        if (wait_abandoned(ws.dg)) break;
         if (spins > kSpinLimit) {
-            if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
+            const unsigned long long mask = __ballot(u == kUnset);
+            const int lane = threadIdx.x & 63;
+            if (lane == __ffsll((long long)mask) - 1) record_timeout(ws, p, u, waited, lane, t0);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -613,7 +660,7 @@ __global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ ybuf, int* __restrict__ counter,
-                                                   int* __restrict__ err, const T* __restrict__ L,
+                                                   SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const int64_t* __restrict__ ioff,
                                                    double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
@@ -622,6 +669,7 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
     const int2 it = items[s_ticket];
+    const WaitSite ws{dg, counter, 1, level, s_ticket, it};
     const DevFront Fr = F[it.x];
     const int rb = it.y, p = Fr.p, m = Fr.m;
     const int r0 = 64 * rb, nbp = min(64, p - r0);
@@ -641,7 +689,7 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
         double lv[16], yv[16];
         const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
         for (int j = 0; j < 16; ++j) lv[j] = lr < nbp ? double(lc[int64_t(j) * m]) : 0.0;
-        poll16(yb + 64 * cb + 16 * q, yv, err);
+        poll16(yb + 64 * cb + 16 * q, yv, ws, cb, 16 * q);
         for (int j = 0; j < 16; ++j) acc += lv[j] * yv[j];
     }
     red[q][lr] = acc;
@@ -753,7 +801,7 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ xbuf, int* __restrict__ counter,
-                                                   int* __restrict__ err, const T* __restrict__ L,
+                                                   SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const int64_t* __restrict__ ioff,
                                                    const double* __restrict__ part, const int64_t* __restrict__ poff,
                                                    double* __restrict__ W, double* __restrict__ x) {
@@ -764,6 +812,7 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
     const int2 it = items[s_ticket];
+    const WaitSite ws2{dg, counter, 2, level, s_ticket, it}, ws3{dg, counter, 3, level, s_ticket, it};
     const DevFront Fr = F[it.x];
     const int cb = it.y, p = Fr.p, m = Fr.m;
     const int np = (p + 63) / 64, nchunk = (m - p + RCH - 1) / RCH;
@@ -802,7 +851,7 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
             const int c = wave + 4 * j;
             lv[j] = (lane < nb2 && c < nbc) ? double(Lf[int64_t(c0 + c) * m + 64 * cb2 + lane]) : 0.0;
         }
-        const double xv = poll1(xb + 64 * cb2 + lane, lane < nb2, err);
+        const double xv = poll1(xb + 64 * cb2 + lane, lane < nb2, ws2, cb2);
         for (int j = 0; j < 16; ++j) acc[j] += lv[j] * xv;
     }
     for (int j = 0; j < 16; ++j) {
@@ -814,7 +863,7 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     double pn = 0.0;
     if (has_next) {
         double xn[16];
-        poll16(xb + 64 * (cb + 1) + 16 * wave, xn, err);
+        poll16(xb + 64 * (cb + 1) + 16 * wave, xn, ws3, cb + 1, 16 * wave);
         for (int j = 0; j < 16; ++j) pn += ln[j] * xn[j];
     }
     red[wave][lane] = pn;
@@ -1179,12 +1228,13 @@ void GpuCholesky<T>::prepare_big_levels() {
     g_xsrc_ = to_device(gxsrc, stream_);
     g_cptr_ = to_device(gcptr, stream_);
     g_cidx_ = to_device(gcidx, stream_);
-    max_flags_ = maxflags;
-    flags_.alloc(size_t(max_flags_) + 2);  // ticket counter at [max_flags_], error word after it
-    xbuf_.alloc(size_t(max_flags_) * 64);
+    max_blocks_ = maxflags;
+    counter_.alloc(1);
+    diag_.alloc(1);
+    xbuf_.alloc(size_t(max_blocks_) * 64);
     part_.alloc(size_t(maxpart));
-    std::vector<int> z(max_flags_ + 2, 0);
-    flags_.upload(z.data(), z.size(), stream_);
+    EMTD_HIP(hipMemsetAsync(counter_.p, 0, sizeof(int), stream_));
+    EMTD_HIP(hipMemsetAsync(diag_.p, 0, sizeof(SweepDiag), stream_));
     EMTD_HIP(hipStreamSynchronize(stream_));
     big_ready_ = true;
 }
@@ -1222,12 +1272,21 @@ void GpuCholesky<T>::invert_diagonal_blocks() {
 }
 
 template <typename T>
-int GpuCholesky<T>::sweep_error() {
-    int h = 0;
-    if (!big_ready_ || !flags_.p) return 0;
-    EMTD_HIP(hipMemcpyAsync(&h, flags_.p + max_flags_ + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
+std::string GpuCholesky<T>::sweep_error() {
+    if (!big_ready_ || !diag_.p) return {};
+    SweepDiag h{};
+    EMTD_HIP(hipMemcpyAsync(&h, diag_.p, sizeof(SweepDiag), hipMemcpyDeviceToHost, stream_));
     EMTD_HIP(hipStreamSynchronize(stream_));
-    return h;
+    if (h.err == 0) return {};
+    static const char* kinds[] = {"?", "forward chain", "backward chain (earlier block)", "backward chain (next block)"};
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  "triangular sweep wait limit reached: %s, level %d, ticket %d of %d taken, front %d block %d "
+                  "waiting for block %d slot %d; polled value 0x%016llx, agent-scope read 0x%016llx, system-scope read "
+                  "0x%016llx (unset = 0x%016llx); waited %.3f ms",
+                  kinds[(h.kind >= 1 && h.kind <= 3) ? h.kind : 0], h.level, h.ticket, h.tickets_taken, h.front, h.block,
+                  h.waited_block, h.slot, h.seen, h.agent_rmw, h.system_rmw, kUnset, double(h.wait_ticks) * 1e-5);
+    return buf;
 }
 
 template <typename T>
@@ -1238,10 +1297,10 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
         W_.alloc(size_t(std::max<int64_t>(S.wsize, 1)) * nrhs);
         wrhs_ = nrhs;
     }
-    int* counter = flags_.p + max_flags_;
-    int* err = flags_.p + max_flags_ + 1;
+    int* counter = counter_.p;
+    SweepDiag* dg = diag_.p;
     unsigned long long* slots = reinterpret_cast<unsigned long long*>(xbuf_.p);
-    const int nslots = max_flags_ * 64;
+    const int nslots = max_blocks_ * 64;
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         const int* lf = level_fronts_.p + S.level_ptr[l];
@@ -1257,7 +1316,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                g_dst_.p + r0, g_bsrc_.p + r0, g_cptr_.p + r0, g_cidx_.p, Wr, d_b + r * ld, slots,
                                nslots, counter);
             hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p,
                                ioff_.p, Wr, d_x + r * ld);
             if (lv_l21_[l + 1] > lv_l21_[l])
                 hipLaunchKernelGGL(k_fwd_l21<T>, dim3(lv_l21_[l + 1] - lv_l21_[l]), dim3(256), 0, stream_, fronts_.p,
@@ -1281,7 +1340,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                 hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(lv_l21t_[l + 1] - lv_l21t_[l]), dim3(256), 0, stream_,
                                    fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
             hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, err, L_.p, inv_.p,
+                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p,
                                ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);
         }
     }

@@ -4,12 +4,23 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "symbolic.hpp"
 
 namespace emtd {
+
+// Written by the first split-sweep wait that reaches its limit (all zero otherwise).
+struct SweepDiag {
+    int err;      // nonzero once a wait gave up; every other wait of the solve then stops
+    int claimed;  // set by the wait that fills in the fields below
+    int kind;     // 1 forward chain, 2 backward chain (earlier block), 3 backward chain (next block)
+    int level, ticket, front, block, waited_block, slot, tickets_taken;
+    unsigned long long seen, agent_rmw, system_rmw;  // the slot: polled, and read by atomic RMWs
+    long long wait_ticks;                            // 100 MHz wall-clock ticks spent waiting
+};
 
 // Per-front descriptor as the kernels read it.
 struct DevFront {
@@ -44,9 +55,8 @@ class GpuCholesky {
 
     // Returns -1 or the first (permuted) pivot that was not positive; synchronizes.
     int check_info();
-    // Nonzero when a large-front sweep hit its wait limit (synchronizes).
-    int sweep_error();
-    // Levels with fewer fronts than this use the multi-workgroup sweeps.
+    // Empty, or the description of the split-sweep wait that hit its limit (synchronizes).
+    std::string sweep_error();
     double last_factor_seconds() const { return factor_seconds_; }
 
     const Symbolic& sym() const { return S_; }
@@ -80,7 +90,8 @@ class GpuCholesky {
     std::vector<int> lv_fc_, lv_l21_, lv_l21t_, lv_bc_;
     DevBuf<int2> fc_items_, l21_items_, bc_items_, inv_items_;
     DevBuf<int3> l21t_items_;
-    DevBuf<int> foff_, flags_;
+    DevBuf<int> foff_, counter_;  // counter_: chain ticket counter
+    DevBuf<SweepDiag> diag_;
     DevBuf<int64_t> poff_, ioff_;
     DevBuf<double> xbuf_, part_;
     // flattened gather rows of the split-sweep levels (per level ranges in lv_rows_)
@@ -89,7 +100,7 @@ class GpuCholesky {
     DevBuf<int> g_xsrc_, g_cptr_;
     DevBuf<T> inv_;
     int n_inv_items_ = 0;
-    int max_flags_ = 1;
+    int max_blocks_ = 1;  // most 64-wide pivot blocks in one split-sweep level
     double factor_seconds_ = 0;
 };
 

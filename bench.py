@@ -98,6 +98,8 @@ def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), s
             info = mg.write_model(spec, d)
             edges += info["n"]
             dirs.append(d)
+            if (i + 1) % 8 == 0 or big:
+                log(f"[bench] generated {i + 1}/{n_requests} models ({info['n']} edges{', large' if big else ''})")
         gaps = [rng.expovariate(rate) if arrivals else 0.0 for _ in dirs]
         srv = emtd.Server(max_active=max_active)
         t0 = time.perf_counter()
@@ -106,6 +108,12 @@ def serving_run(n_requests, rate, seed, big_fraction=0.0, axis_range=(24, 64), s
             if g > 0:
                 time.sleep(g)
             ids.append(srv.submit(d))
+        log(f"[bench] all {n_requests} requests submitted after {time.perf_counter() - t0:.1f} s")
+        for k, rid in enumerate(ids):   # progress as requests complete (in submission order)
+            inf = srv.wait(rid, check=False)
+            why = (": " + emtd.lib().emtd_last_error().decode(errors="replace")) if inf.rc else ""
+            log(f"[bench] request {k}: status {inf.status} rc {inf.rc} edges {inf.n_edges} "
+                f"latency {inf.end_s - inf.submit_s:.2f} s{why}")
         srv.drain()
         wall = time.perf_counter() - t0
         infos = [srv.poll(i) for i in ids]
