@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 
 #include "chol.hpp"
 
@@ -349,12 +350,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Stages the nb x nb diagonal block L[c0.., c0..] of a front into LDS (D[row][col]) with
-// coalesced column reads, so the sequential triangular solve reads LDS instead of HBM.
+// coalesced column reads, so the sequential triangular solve reads LDS instead of HBM. For
+// 256-thread workgroups: each thread issues its 16 loads before the first LDS store, so the
+// staging costs one memory round trip rather than 16.
 template <typename T>
 __device__ __forceinline__ void stage_diag(double (*D)[SB + 1], const T* Lf, int m, int c0, int nb) {
-    for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x) {
+    constexpr int PER = SB * SB / 256;
+    double t[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + 256 * k;
         const int i = idx & (SB - 1), j = idx / SB;
-        D[i][j] = (i < nb && j < nb && i >= j) ? double(Lf[int64_t(c0 + j) * m + c0 + i]) : 0.0;
+        t[k] = (i < nb && j < nb && i >= j) ? double(Lf[int64_t(c0 + j) * m + c0 + i]) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        D[idx & (SB - 1)][idx / SB] = t[k];
     }
 }
 
@@ -479,6 +491,191 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
         __syncthreads();
     }
     for (int i = threadIdx.x; i < p; i += blockDim.x) xx[Fr.fbeg + i] = v[i];
+}
+
+// ------------------------------------------------------------------ sweeps of small fronts
+// Levels whose fronts all have at most 64 pivots (the many small fronts near the leaves) use one
+// wavefront per front, four fronts per workgroup: the pivot block L_bb lives in registers and
+// every global access of a wavefront is issued in one batch, so a front costs a few memory
+// round trips and the level runs at streaming bandwidth instead of being bound by per-front
+// latency.
+constexpr int kSmallP = 64;
+
+__device__ __forceinline__ double bcast_lane(double v, int j) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(u), j);
+    const int hi = __builtin_amdgcn_readlane(int(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Forward: v (m rows, in LDS) = b on the pivot rows + the children's update entries (child by
+// child, fixed order); y = L_bb^-1 v_P by substitution with lane i holding row i of L_bb
+// (coalesced column loads); then v_U -= L21 y, 64 rows per pass with all column loads of the
+// pass in flight. y goes to x, v_U to the workspace for the parent.
+// Column c of a front's L block (uniform across the wavefront: a scalar base, so that the
+// loads of a column use one 32-bit lane offset instead of a 64-bit address per lane).
+template <typename T>
+__device__ __forceinline__ const T* colp(const T* Lf, int c, int m) {
+    return Lf + int64_t(c) * m;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_small(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                   int nfl, int mcap, const int* __restrict__ children,
+                                                   const int* __restrict__ relmap, const T* __restrict__ L,
+                                                   double* __restrict__ W, int64_t wsize,
+                                                   const double* __restrict__ b, double* __restrict__ x, int64_t ld) {
+    extern __shared__ double smem[];
+    // the wavefront index is uniform: readfirstlane lets the compiler keep the front's
+    // descriptor, pointers and sizes in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int fi = blockIdx.x * 4 + wave;
+    if (fi >= nfl) return;
+    const int rhs = blockIdx.y;
+    double* vs = smem + wave * mcap;
+    const DevFront Fr = F[flist[fi]];
+    const int p = Fr.p, m = Fr.m;
+    const double* bb = b + rhs * ld;
+    double* Wr = W + rhs * wsize;
+    const T* Lf = L + Fr.loff;
+    // pivot block rows (strictly lower part) and diagonal, loaded first so that they are in
+    // flight during the gather
+    double dreg[kSmallP];
+#pragma unroll
+    for (int j = 0; j < kSmallP; ++j) dreg[j] = (j < p && j < lane && lane < p) ? double(colp(Lf, j, m)[unsigned(lane)]) : 0.0;
+    const double dg = lane < p ? double(Lf[int64_t(lane) * m + lane]) : 1.0;
+    // rows [m, 64) are zeroed too (mcap >= 64): the rows pass below reads vs[0..64) whole
+    for (int i = lane; i < max(m, kSmallP); i += 64) vs[i] = i < p ? bb[Fr.fbeg + i] : 0.0;
+    for (int ci = Fr.child_beg; ci < Fr.child_end; ++ci) {
+        const DevFront Fc = F[children[ci]];
+        const int uc = Fc.m - Fc.p;
+        const int* rm = relmap + Fc.relmap_off;
+        const double* wc = Wr + Fc.woff + Fc.p;
+        for (int k0 = 0; k0 < uc; k0 += 256) {
+            int ri[4];
+            double wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + 64 * q + lane;
+                ri[q] = k < uc ? rm[k] : -1;
+                wv[q] = k < uc ? wc[k] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ri[q] >= 0) vs[ri[q]] += wv[q];
+        }
+    }
+    const double rdg = 1.0 / dg;
+    double val = lane < p ? vs[lane] : 0.0;
+    double y = 0.0;
+#pragma unroll
+    for (int j = 0; j < kSmallP; ++j) {
+        if (j < p) {
+            const double yj = bcast_lane(val * rdg, j);
+            if (lane == j) y = yj;
+            val = fma(-dreg[j], yj, val);
+        }
+    }
+    if (lane < p) {
+        vs[lane] = y;
+        x[rhs * ld + Fr.fbeg + lane] = y;
+    }
+    for (int r0 = p; r0 < m; r0 += 64) {
+        const unsigned r = unsigned(r0 + lane);
+        const bool rin = r < unsigned(m);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        // 32 columns at a time; y from LDS 16 values at a time (broadcast reads, one wait per
+        // group); columns past p carry lv = 0 and their slots hold finite row values or zeros
+#pragma unroll
+        for (int c0 = 0; c0 < kSmallP; c0 += 32) {
+            if (c0 < p) {
+                double lv[32];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) lv[j] = (c0 + j < p && rin) ? double(colp(Lf, c0 + j, m)[r]) : 0.0;
+#pragma unroll
+                for (int j0 = 0; j0 < 32; j0 += 16) {
+                    double yv[16];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) yv[q] = vs[c0 + j0 + q];
+#pragma unroll
+                    for (int q = 0; q < 16; q += 4) {
+                        a0 = fma(lv[j0 + q], yv[q], a0);
+                        a1 = fma(lv[j0 + q + 1], yv[q + 1], a1);
+                        a2 = fma(lv[j0 + q + 2], yv[q + 2], a2);
+                        a3 = fma(lv[j0 + q + 3], yv[q + 3], a3);
+                    }
+                }
+            }
+        }
+        if (rin) Wr[Fr.woff + r] = vs[r] - ((a0 + a1) + (a2 + a3));
+    }
+}
+
+// Backward: s = L21^T x_U with lanes over rows and 32 column sums per lane, reduced across the
+// wavefront by a butterfly that leaves column c in lane c; then x_P = L_bb^-T (y - s) by
+// substitution with lane c holding column c of L_bb.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_small(const DevFront* __restrict__ F, const int* __restrict__ flist,
+                                                   int nfl, const int* __restrict__ rows, const T* __restrict__ L,
+                                                   double* __restrict__ x, int64_t ld) {
+    // the wavefront index is uniform: readfirstlane lets the compiler keep the front's
+    // descriptor, pointers and sizes in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int fi = blockIdx.x * 4 + wave;
+    if (fi >= nfl) return;
+    double* xx = x + blockIdx.y * ld;
+    const DevFront Fr = F[flist[fi]];
+    const int p = Fr.p, m = Fr.m;
+    const T* Lf = L + Fr.loff;
+    const int* rw = rows + Fr.rows_off;
+    double s_own = 0.0;
+    for (int h = 0; h < p; h += 32) {
+        double acc[32];
+#pragma unroll
+        for (int c = 0; c < 32; ++c) acc[c] = 0.0;
+        for (int r0 = p; r0 < m; r0 += 64) {
+            const unsigned r = unsigned(r0 + lane);
+            const bool rin = r < unsigned(m);
+            const double xr = rin ? xx[rw[r]] : 0.0;
+            double lv[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c) lv[c] = (h + c < p && rin) ? double(colp(Lf, h + c, m)[r]) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) acc[c] = fma(lv[c], xr, acc[c]);
+        }
+        // sum over the 64 lanes; afterwards lane l holds column h + (l & 31)
+#pragma unroll
+        for (int c = 0; c < 32; ++c) acc[c] += __shfl_xor(acc[c], 32, 64);
+#pragma unroll
+        for (int o = 16, n = 32; o >= 1; o >>= 1, n >>= 1) {
+            const bool up = (lane & o) != 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (k < n / 2) {
+                    const double send = up ? acc[k] : acc[k + n / 2];
+                    const double keep = up ? acc[k + n / 2] : acc[k];
+                    acc[k] = keep + __shfl_xor(send, o, 64);
+                }
+            }
+        }
+        if ((lane >> 5) == (h >> 5)) s_own = acc[0];
+    }
+    double dcol[kSmallP];
+#pragma unroll
+    for (int j = 0; j < kSmallP; ++j) dcol[j] = (j < p && j > lane) ? double(Lf[int64_t(lane) * m + j]) : 0.0;
+    const double dg = lane < p ? double(Lf[int64_t(lane) * m + lane]) : 1.0;
+    const double rdg = 1.0 / dg;
+    double val = lane < p ? xx[Fr.fbeg + lane] - s_own : 0.0;
+    double xo = 0.0;
+#pragma unroll
+    for (int j = kSmallP - 1; j >= 0; --j) {
+        if (j < p) {
+            const double xj = bcast_lane(val * rdg, j);
+            if (lane == j) xo = xj;
+            val = fma(-dcol[j], xj, val);
+        }
+    }
+    if (lane < p) xx[Fr.fbeg + lane] = xo;
 }
 
 // ------------------------------------------------------------------ sweeps of large fronts
@@ -641,7 +838,15 @@ __global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, 
     const int c0 = 64 * it.y, nb = min(64, Fr.p - c0);
     const T* Lf = L + Fr.loff;
     const int t = threadIdx.x;
-    for (int j = 0; j < SB; ++j) A[t][j] = (t < nb && j < nb && t >= j) ? double(Lf[int64_t(c0 + j) * Fr.m + c0 + t]) : 0.0;
+    // 16 loads in flight per thread
+    for (int j0 = 0; j0 < SB; j0 += 16) {
+        double r[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            r[j] = (t < nb && j0 + j < nb && t >= j0 + j) ? double(Lf[int64_t(c0 + j0 + j) * Fr.m + c0 + t]) : 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) A[t][j0 + j] = r[j];
+    }
     __syncthreads();
     // column t of the inverse by forward substitution
     for (int i = 0; i < SB; ++i) {
@@ -1135,6 +1340,12 @@ template <typename T>
 void GpuCholesky<T>::prepare_big_levels() {
     const Symbolic& S = S_;
     big_.assign(S.nlevels, 0);
+    lv_mcap_.assign(S.nlevels, 0);
+    // sweep strategy overrides (tuning and tests): levels with fewer fronts than
+    // EMTD_SPLIT_LEVEL_FRONTS use the split sweeps (0: none); EMTD_SMALL_FRONT_SWEEPS=0 turns off
+    // the one-wavefront-per-front sweeps
+    if (const char* e = std::getenv("EMTD_SPLIT_LEVEL_FRONTS")) big_level_fronts_ = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("EMTD_SMALL_FRONT_SWEEPS")) small_sweeps_ = std::atoi(e) != 0;
     std::vector<int2> fc, l21, bc;
     std::vector<int3> l21t;
     std::vector<int> foff(S.nf, 0);
@@ -1151,6 +1362,14 @@ void GpuCholesky<T>::prepare_big_levels() {
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         big_[l] = nfl < big_level_fronts_ ? 1 : 0;
+        // one wavefront per front when every front has at most kSmallP pivots and four fronts'
+        // rows fit in 48 KB of LDS
+        int maxp = 0, maxm = 0;
+        for (int k = S.level_ptr[l]; k < S.level_ptr[l + 1]; ++k) {
+            maxp = std::max(maxp, S.pcount[S.level_fronts[k]]);
+            maxm = std::max(maxm, S.mcount[S.level_fronts[k]]);
+        }
+        lv_mcap_[l] = (small_sweeps_ && !big_[l] && maxp <= kSmallP && maxm <= 1536) ? std::max(maxm, kSmallP) : 0;
         if (big_[l]) {
             int off = 0, maxnp = 0;
             int64_t po = 0;
@@ -1304,6 +1523,12 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         const int* lf = level_fronts_.p + S.level_ptr[l];
+        if (!big_[l] && lv_mcap_[l] > 0) {
+            hipLaunchKernelGGL(k_fwd_small<T>, dim3((nfl + 3) / 4, nrhs), dim3(256), size_t(4) * lv_mcap_[l] * 8,
+                               stream_, fronts_.p, lf, nfl, lv_mcap_[l], children_.p, relmap_.p, L_.p, W_.p, S.wsize,
+                               d_b, d_x, ld);
+            continue;
+        }
         if (!big_[l]) {
             hipLaunchKernelGGL(k_fwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p, lf, children_.p,
                                relmap_.p, L_.p, W_.p, S.wsize, d_b, d_x, ld);
@@ -1326,6 +1551,11 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
     for (int l = S.nlevels - 1; l >= 0; --l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
         const int* lf = level_fronts_.p + S.level_ptr[l];
+        if (!big_[l] && lv_mcap_[l] > 0) {
+            hipLaunchKernelGGL(k_bwd_small<T>, dim3((nfl + 3) / 4, nrhs), dim3(256), 0, stream_, fronts_.p, lf, nfl,
+                               rows_.p, L_.p, d_x, ld);
+            continue;
+        }
         if (!big_[l]) {
             hipLaunchKernelGGL(k_bwd<T>, dim3(nfl, nrhs), dim3(256), 0, stream_, fronts_.p, lf, rows_.p, L_.p, W_.p,
                                S.wsize, d_x, ld);

@@ -84,9 +84,11 @@ class GpuCholesky {
     std::vector<int> level_sorted_off_;
     DevBuf<int> info_, panel_done_;
     int wrhs_ = 0;
-    int big_level_fronts_ = 1024;
+    int big_level_fronts_ = 1024;  // levels with fewer fronts use the split sweeps
+    bool small_sweeps_ = true;     // one wavefront per front where every front has <= 64 pivots
     bool big_ready_ = false;
     std::vector<char> big_;
+    std::vector<int> lv_mcap_;  // > 0: the level uses the one-wavefront-per-front sweeps (max rows)
     std::vector<int> lv_fc_, lv_l21_, lv_l21t_, lv_bc_;
     DevBuf<int2> fc_items_, l21_items_, bc_items_, inv_items_;
     DevBuf<int3> l21t_items_;

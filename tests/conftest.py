@@ -52,3 +52,19 @@ def make_model(tmpdir, cfg=1, cells=(8, 8, 6), steps=(4, 3), t_end=1e-3, current
     d = str(tmpdir)
     m.write_model(spec, d)
     return d, spec
+
+
+# Sweep strategies: "split" is the default (levels with fewer than 1024 fronts split over many
+# workgroups, which at these sizes is every level); "small" turns the split sweeps off so that
+# the levels whose fronts have at most 64 pivots run one wavefront per front and the others one
+# workgroup per front; "classic" turns both off (one workgroup per front everywhere).
+SWEEPS = {"split": {}, "small": {"EMTD_SPLIT_LEVEL_FRONTS": "0"},
+          "classic": {"EMTD_SPLIT_LEVEL_FRONTS": "0", "EMTD_SMALL_FRONT_SWEEPS": "0"}}
+
+
+@pytest.fixture(params=list(SWEEPS))
+def sweeps(request, monkeypatch):
+    for k, v in SWEEPS[request.param].items():
+        monkeypatch.setenv(k, v)
+    print("sweeps", request.param, flush=True)
+    return request.param

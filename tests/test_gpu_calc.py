@@ -87,6 +87,21 @@ def test_calc_matches_oracle(emtd, tmp_path, form, scheme, current):
         assert err < TOL64, stem
 
 
+def test_calc_sweep_strategies_match_oracle(emtd, tmp_path, sweeps):
+    """The A 3-point case through each sweep strategy: receivers vs the oracle at the FP64 bar."""
+    d, spec = make_model(tmp_path, cfg=3, cells=(10, 10, 8), steps=(4, 3), form="A", scheme=3)
+    m = refio.read_model(d)
+    ref = calc.run(m, "A", 3)
+    c = emtd.Calc(d, 0, 3, emtd.FP64)
+    c.run()
+    for kind, want in ((0, ref.valB), (1, ref.valE)):
+        got = c.receiver_values(kind)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print(sweeps, "kind", kind, "receiver max rel err", err)
+        assert err < TOL64
+    c.close()
+
+
 def test_calc_drop_in_executable(emtd, tmp_path):
     """The drop-in binary reads the directory and writes the three tables."""
     import subprocess
@@ -102,7 +117,7 @@ TOL32 = 1e-5
 
 
 @pytest.mark.parametrize("form,scheme", [("A", 2), ("E", 3)])
-def test_calc_fp32_refined_matches_fp64(emtd, tmp_path, form, scheme):
+def test_calc_fp32_refined_matches_fp64(emtd, tmp_path, form, scheme, sweeps):
     """FP32 factor + FP64 refinement: receivers within 1e-5 of the FP64 path and the oracle."""
     d, _ = make_model(tmp_path, cfg=3, cells=(10, 10, 8), steps=(5, 3), form=form, scheme=scheme)
     print("case fp32-ir", form, scheme, flush=True)

@@ -62,7 +62,7 @@ def _solve_both(emtd, d, sigma_floor, nrhs=2):
 
 
 @pytest.mark.parametrize("cells", [(6, 6, 4), (10, 10, 8), (14, 12, 10)])
-def test_solver_fp64_well_conditioned(emtd, tmp_path, cells):
+def test_solver_fp64_well_conditioned(emtd, tmp_path, cells, sweeps):
     """Conductivity floored at 1e-2 S/m: the system is well conditioned, tight bar."""
     d, _ = make_model(tmp_path, cfg=1, cells=cells)
     res, res_ref, err = _solve_both(emtd, d, 1e-2)
@@ -71,7 +71,7 @@ def test_solver_fp64_well_conditioned(emtd, tmp_path, cells):
 
 
 @pytest.mark.parametrize("cells", [(6, 6, 4), (14, 12, 10)])
-def test_solver_fp64_with_air(emtd, tmp_path, cells):
+def test_solver_fp64_with_air(emtd, tmp_path, cells, sweeps):
     """Real air (1e-8 S/m): kappa ~ 1e8-1e10; the residual must match a backward-stable
     direct solve (SuperLU's) and the solution SuperLU's to its conditioning."""
     d, _ = make_model(tmp_path, cfg=1, cells=cells)
