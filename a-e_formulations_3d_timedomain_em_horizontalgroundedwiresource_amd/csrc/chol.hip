@@ -90,7 +90,7 @@ __device__ __forceinline__ T* front_elem(const DevFront& F, T* L, T* U, int r, i
 template <typename T>
 __global__ void k_extend_add(const DevFront* __restrict__ F, const int2* __restrict__ items, int nitems,
                              const int* __restrict__ relmap, T* __restrict__ L, T* __restrict__ U) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int it = blockIdx.x * (blockDim.x >> 6) + wave;
     if (it >= nitems) return;
     const int c = items[it].x, b = items[it].y;
@@ -120,6 +120,17 @@ __device__ __forceinline__ double rsqrt_newton(double d) {
 __device__ __forceinline__ float rsqrt_newton(float d) {
     float y = __builtin_amdgcn_rsqf(d);
     return y * (1.5f - 0.5f * d * y * y);
+}
+
+// Value of v in lane j of the wavefront (j uniform).
+__device__ __forceinline__ double bcast_lane(double v, int j) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(u), j);
+    const int hi = __builtin_amdgcn_readlane(int(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float bcast_lane(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
 // One panel step of the blocked factorization, for the columns [kb, kb+nb) of each active
@@ -154,19 +165,16 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
         a[j] = (diag && j < nb && t >= j) ? col[j * m + kb + t] : T(0);
         x[j] = (row && j < nb) ? col[j * m + r] : T(0);
     }
-    // Right-looking. Every lane keeps the running diagonal dd[l] = A[l][l] - sum_k L[l][k]^2, so
-    // the pivot needs no communication; its reciprocal square root (hardware estimate refined
-    // by two Newton steps to full precision) scales column j, which is published in LDS (two
-    // buffers alternate, so one barrier per pivot suffices) and applied by every lane to its
-    // diagonal-block row, its panel row and its copy of the diagonal.
-    T dd[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) dd[j] = j < nb ? col[j * m + kb + j] : T(1);
+    // Right-looking. Lane j's a[j] is the running pivot A[j][j] - sum_k L[j][k]^2 (its own
+    // diagonal entry after the previous rank-1 updates), read by every lane with readlane; its
+    // reciprocal square root (hardware estimate refined by two Newton steps to full precision)
+    // scales column j, which is published in LDS (two buffers alternate, so one barrier per
+    // pivot suffices) and applied by every lane to its diagonal-block row and its panel row.
     int bad = -1;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (j < nb) {
-            T d = dd[j];
+            T d = bcast_lane(a[j], j);
             if (!(d > T(0))) {
                 if (bad < 0) bad = j;
                 d = T(1);
@@ -177,14 +185,11 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
             if (diag && t >= j) colj[j & 1][t] = a[j];  // column j of the factor, rows j..nb-1
             __syncthreads();
             x[j] *= y;
-            T cc[NB];
-#pragma unroll
-            for (int l = j + 1; l < NB; ++l) cc[l] = colj[j & 1][l];
 #pragma unroll
             for (int l = j + 1; l < NB; ++l) {
-                dd[l] -= cc[l] * cc[l];
-                a[l] -= a[j] * cc[l];
-                x[l] -= x[j] * cc[l];
+                const T c = colj[j & 1][l];
+                a[l] -= a[j] * c;
+                x[l] -= x[j] * c;
             }
         }
     }
@@ -242,7 +247,9 @@ struct Mfma<float> {
 // flight during the MFMAs, and the LDS fragments of the next k-step are read before the
 // current k-step's MFMAs. The MFMA computes the transposed block (column operand first), so a
 // lane's accumulators map to 16 consecutive rows of C and the read-modify-write of C is
-// coalesced along columns.
+// coalesced along columns. Register budget: the 1024-thread variant must stay within 64 VGPRs,
+// or only one workgroup fits per CU (measured: preloading C into the accumulators took it to
+// 68 and the factorization from 137 to 157 ms at 1M DOF).
 constexpr int UPAD = 4;
 
 template <typename T, int TM, int WV, int KC>
@@ -265,7 +272,7 @@ __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restr
     if (R0 >= m || C0 >= ce) return;
     __shared__ T Rs[2][KC][TM + UPAD];
     __shared__ T Cs[2][KC][TM + UPAD];
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int t = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
     const int wr = (wave / WV) * WT, wc = (wave % WV) * WT;
     const bool skip = (ti == tj && wr < wc);  // strictly upper block of a diagonal tile
     const int li = lane & 15, lk = lane >> 4;
@@ -453,7 +460,7 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
     __shared__ double rdg[SB];
     __shared__ double D[SB][SB + 1];
     const T* Lf = L + Fr.loff;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     for (int kb = ((p - 1) / SB) * SB; kb >= 0; kb -= SB) {
         const int nb = min(SB, p - kb);
         // s[c] = sum over the rows below the block of L[r][kb+c] v[r]: wave w owns the columns
@@ -500,13 +507,6 @@ __global__ void __launch_bounds__(256) k_bwd(const DevFront* __restrict__ F, con
 // round trips and the level runs at streaming bandwidth instead of being bound by per-front
 // latency.
 constexpr int kSmallP = 64;
-
-__device__ __forceinline__ double bcast_lane(double v, int j) {
-    const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane(int(u), j);
-    const int hi = __builtin_amdgcn_readlane(int(u >> 32), j);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
 
 // Forward: v (m rows, in LDS) = b on the pivot rows + the children's update entries (child by
 // child, fixed order); y = L_bb^-1 v_P by substitution with lane i holding row i of L_bb
@@ -878,7 +878,7 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     const DevFront Fr = F[it.x];
     const int rb = it.y, p = Fr.p, m = Fr.m;
     const int r0 = 64 * rb, nbp = min(64, p - r0);
-    const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int lr = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = r0 + lr;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
@@ -926,7 +926,7 @@ __global__ void __launch_bounds__(256) k_fwd_l21(const DevFront* __restrict__ F,
     const int2 it = items[blockIdx.x];
     const DevFront Fr = F[it.x];
     const int p = Fr.p, m = Fr.m;
-    const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int lr = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = p + 64 * it.y + lr;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
@@ -962,7 +962,7 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
     const int p = Fr.p, m = Fr.m;
     const int c0 = 64 * it.y, nbc = min(64, p - c0);
     const int rb = p + RCH * it.z, re = min(m, rb + RCH);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const T* Lf = L + Fr.loff;
     const double* v = W + Fr.woff;
     double acc[16];
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     const int cb = it.y, p = Fr.p, m = Fr.m;
     const int np = (p + 63) / 64, nchunk = (m - p + RCH - 1) / RCH;
     const int c0 = 64 * cb, nbc = min(64, p - c0);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
@@ -1180,13 +1180,13 @@ GpuCholesky<T>::~GpuCholesky() {
 
 template <typename T>
 void GpuCholesky<T>::update(hipStream_t st, const int* fz, int cnt, int k0, int k1, int cmin, int cmax, int rows,
-                            int cols) {
+                            int cols, size_t lds_pad) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
     if (cols >= 256) {
         const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
-        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), 0, st, fronts_.p, fz, k0, k1,
-                           cmin, cmax, L_.p, U_.p);
+        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), lds_pad, st, fronts_.p, fz, k0,
+                           k1, cmin, cmax, L_.p, U_.p);
     } else {
         const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
         hipLaunchKernelGGL((k_update<T, 64, 2, KC>), dim3(tr, tc, cnt), dim3(256), 0, st, fronts_.p, fz, k0, k1, cmin,
@@ -1218,6 +1218,13 @@ void GpuCholesky<T>::numeric() {
         EMTD_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
         EMTD_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     }
+    // The bulk update on the second stream asks for extra LDS so that at most one of its
+    // 1024-thread workgroups runs per CU: the other half of each CU's registers stays free for
+    // the panel chain of the main stream, which otherwise waits for the bulk update to drain.
+    if (const char* e = std::getenv("EMTD_LOOKAHEAD_LDS_PAD")) side_lds_pad_ = size_t(std::max(0, std::atoi(e)));
+    // outer block (a multiple of the panel width; tuning override EMTD_OUTER_BLOCK)
+    int nbo = NBO;
+    if (const char* e = std::getenv("EMTD_OUTER_BLOCK")) nbo = std::max(1, std::atoi(e) / NB) * NB;
     bool side_pending = false;
     auto join_side = [&]() {
         if (!side_pending) return;
@@ -1231,7 +1238,7 @@ void GpuCholesky<T>::numeric() {
         const int* dfl = level_sorted_dev_.p + level_sorted_off_[l];
         const int maxp = S.pcount[fl[0]];
         int nact = int(fl.size());
-        for (int kb = 0; kb < maxp; kb += NBO) {
+        for (int kb = 0; kb < maxp; kb += nbo) {
             while (nact > 0 && S.pcount[fl[nact - 1]] <= kb) --nact;
             if (nact == 0) break;
             int mm = 0, mp = 0;
@@ -1242,25 +1249,25 @@ void GpuCholesky<T>::numeric() {
             for (int nz = 0; nz < nact; nz += 65535) {
                 const int cnt = std::min(65535, nact - nz);
                 const int* fz = dfl + nz;
-                for (int ks = kb; ks < std::min(kb + NBO, maxp); ks += NB) {
+                for (int ks = kb; ks < std::min(kb + nbo, maxp); ks += NB) {
                     const int rem = mm - ks - 1;
                     hipLaunchKernelGGL(k_panel<T>, dim3(std::max(1, (rem + 63) / 64), cnt), dim3(64), 0, stream_,
                                        fronts_.p, fz, ks, L_.p, info_.p, panel_done_.p);
-                    // inside the outer block: columns [ks+NB, kb+NBO) with K = NB
-                    update(stream_, fz, cnt, ks, ks + NB, 0, kb + NBO, rem, std::min(kb + NBO, mp) - (ks + 1));
+                    // inside the outer block: columns [ks+NB, kb+nbo) with K = NB
+                    update(stream_, fz, cnt, ks, ks + NB, 0, kb + nbo, rem, std::min(kb + nbo, mp) - (ks + 1));
                 }
                 // the previous bulk update wrote the columns the trailing updates touch
                 join_side();
-                const int k1 = kb + NBO, k2 = kb + 2 * NBO;
-                // the remaining pivot columns [kb+2NBO, p) on the side stream, once the block's
+                const int k1 = kb + nbo, k2 = kb + 2 * nbo;
+                // the remaining pivot columns [kb+2nbo, p) on the side stream, once the block's
                 // columns are final
                 if (mp > k2) {
                     EMTD_HIP(hipEventRecord(ev_main_, stream_));
                     EMTD_HIP(hipStreamWaitEvent(side_, ev_main_, 0));
-                    update(side_, fz, cnt, kb, k1, k2, INT_MAX, mm - k2, mp - k2);
+                    update(side_, fz, cnt, kb, k1, k2, INT_MAX, mm - k2, mp - k2, side_lds_pad_);
                     side_pending = true;
                 }
-                // next outer block's columns [kb+NBO, kb+2NBO), K = NBO, all rows
+                // next outer block's columns [kb+nbo, kb+2nbo), K = nbo, all rows
                 update(stream_, fz, cnt, kb, k1, 0, k2, mm - k1, std::min(k2, mp) - k1);
             }
         }

@@ -64,7 +64,8 @@ class GpuCholesky {
 
   private:
     void numeric();
-    void update(hipStream_t st, const int* fz, int cnt, int k0, int k1, int cmin, int cmax, int rows, int cols);
+    void update(hipStream_t st, const int* fz, int cnt, int k0, int k1, int cmin, int cmax, int rows, int cols,
+                size_t lds_pad = 0);
     void extend_add(int l, const std::vector<int>& ptr, const int2* items);
     void prepare_big_levels();
     void prepare_inverses();
@@ -85,6 +86,7 @@ class GpuCholesky {
     DevBuf<int> info_, panel_done_;
     int wrhs_ = 0;
     int big_level_fronts_ = 1024;  // levels with fewer fronts use the split sweeps
+    size_t side_lds_pad_ = 0;       // extra LDS of the look-ahead bulk update (one workgroup per CU)
     bool small_sweeps_ = true;     // one wavefront per front where every front has <= 64 pivots
     bool big_ready_ = false;
     std::vector<char> big_;
