@@ -40,7 +40,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *COMMON, "-shared", "-pthread", "-o", tmp, *objs]
+    cmd = [HIPCC, *COMMON, "-shared", "-pthread", "-o", tmp, *objs, "-lrt"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -80,7 +80,26 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
     perm_ = to_device(S_->perm, stream_);
     if (precision_ == EMTD_FP32_IR) build_residual_operator();
-    // C in permuted full CSR (exact zeros dropped) for the RHS
+    // primary/secondary mode (A2/time_approx.cpp:350-370, 456-468): the (sigma - sigma0) mass
+    // matrix C_dsig, Assembling(4) of the reference, on the same pattern
+    ps_ = M_.n_anom > 0 && !M_.fdirect;
+    std::vector<double> dg, dd;
+    if (ps_) {
+        if (M_.pname.empty()) throw Error(EMTD_ERR_NOFILE, "mtrsizeforE0 given but PName missing");
+        std::vector<double> dsig(M_.nmat);
+        for (int m = 0; m < M_.nmat; ++m) dsig[m] = M_.sigma[m] - M_.sigma0[m];
+        emtd_mesh md = mesh;
+        md.sigma = dsig.data();
+        DevBuf<double> tb(std::max<int64_t>(nnz_, 1)), tbd(n_), tc(std::max<int64_t>(nnz_, 1)), tcd(n_);
+        Assembler A(md, n_, ig_.data(), jg_.data(), stream_);
+        A.run(tb.p, tbd.p, tc.p, tcd.p);
+        dg.resize(nnz_);
+        dd.resize(n_);
+        tc.download(dg.data(), nnz_, stream_);
+        tcd.download(dd.data(), n_, stream_);
+        EMTD_HIP(hipStreamSynchronize(stream_));
+    }
+    // C (and C_dsig) in permuted full CSR for the RHS; entries zero in both are dropped
     {
         std::vector<double> cg(nnz_), cd(n_);
         c_ggl_.download(cg.data(), nnz_, stream_);
@@ -88,29 +107,62 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         EMTD_HIP(hipStreamSynchronize(stream_));
         c_host_ggl_ = cg;
         c_host_di_ = cd;
-        std::vector<std::vector<std::pair<int, double>>> rows(n_);
+        struct Ent {
+            int col;
+            double c, d;
+            bool operator<(const Ent& o) const { return col < o.col; }
+        };
+        std::vector<std::vector<Ent>> rows(n_);
         for (int i = 0; i < n_; ++i) {
             const int a = S_->iperm[i];
-            if (cd[i] != 0.0) rows[a].push_back({a, cd[i]});
+            const double di = ps_ ? dd[i] : 0.0;
+            if (cd[i] != 0.0 || di != 0.0) rows[a].push_back({a, cd[i], di});
             for (int k = ig_[i]; k < ig_[i + 1]; ++k) {
-                if (cg[k] == 0.0) continue;
+                const double dk = ps_ ? dg[k] : 0.0;
+                if (cg[k] == 0.0 && dk == 0.0) continue;
                 const int b = S_->iperm[jg_[k]];
-                rows[a].push_back({b, cg[k]});
-                rows[b].push_back({a, cg[k]});
+                rows[a].push_back({b, cg[k], dk});
+                rows[b].push_back({a, cg[k], dk});
             }
         }
         std::vector<int64_t> rp(n_ + 1, 0);
         std::vector<int> ci;
-        std::vector<double> cv;
+        std::vector<double> cv, cdv;
         for (int a = 0; a < n_; ++a) {
             std::sort(rows[a].begin(), rows[a].end());
-            for (auto& e : rows[a]) { ci.push_back(e.first); cv.push_back(e.second); }
+            for (auto& e : rows[a]) {
+                ci.push_back(e.col);
+                cv.push_back(e.c);
+                if (ps_) cdv.push_back(e.d);
+            }
             rp[a + 1] = int64_t(ci.size());
         }
         crp_ = to_device(rp, stream_);
         cci_ = to_device(ci, stream_);
         ccv_ = to_device(cv, stream_);
+        if (ps_) cdv_ = to_device(cdv, stream_);
         c_nnz_ = int64_t(ci.size());
+    }
+    if (ps_) {
+        for (int i = 0; i < n_; ++i) {
+            if (!M_.anomal[i]) continue;
+            const int v1 = M_.edges[2 * i], v2 = M_.edges[2 * i + 1];
+            double len = 0;
+            for (int a = 0; a < 3; ++a) {
+                const double d = M_.xyz[3 * v2 + a] - M_.xyz[3 * v1 + a];
+                len += d * d;
+            }
+            anom_edges_.push_back(S_->iperm[i]);
+            anom_half_.push_back(std::sqrt(len) * 0.5);
+        }
+        anom_idx_ = to_device(anom_edges_, stream_);
+        en_.alloc(size_t(n_) * npls_);
+        EMTD_HIP(hipMemsetAsync(en_.p, 0, en_.n * sizeof(double), stream_));
+        en_vals_.alloc(size_t(M_.n_anom) * npls_);
+        for (int k = 0; k < 2; ++k) {
+            EMTD_HIP(hipHostMalloc((void**)&en_stage_[k], en_vals_.n * sizeof(double), hipHostMallocDefault));
+            EMTD_HIP(hipEventCreateWithFlags(&en_ev_[k], hipEventDisableTiming));
+        }
     }
     // initial field (A: nodal a0 projected on edges; E: -grad V), permuted
     {
@@ -188,6 +240,13 @@ CalcCore::~CalcCore() {
         c64_.reset();
         c32_.reset();
         stored_.clear();
+        for (int k = 0; k < 2; ++k) {
+            if (en_ev_[k]) {
+                (void)hipEventSynchronize(en_ev_[k]);
+                (void)hipEventDestroy(en_ev_[k]);
+            }
+            if (en_stage_[k]) (void)hipHostFree(en_stage_[k]);
+        }
         for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
         if (stream_) (void)hipStreamDestroy(stream_);
     } catch (...) {
@@ -205,6 +264,8 @@ int64_t CalcCore::estimate_footprint() const {
     b += (2 * nnz + n) * 12 + n * 8;      // C in permuted CSR
     b += n * 4;                           // permutation
     b += 12 * n * np * 8;                 // fields, RHS, work vectors, kept layers
+    if (M_.n_anom > 0 && !M_.fdirect)
+        b += (2 * nnz + n) * 8 + n * np * 8 + int64_t(M_.n_anom) * (np * 8 + 4);  // C_dsig, E_n
     if (precision_ == EMTD_FP64) {
         b += factor(8);
     } else {
@@ -407,6 +468,66 @@ double CalcCore::source_scale(int itime, double dt_two_layer) const {
     return -djdt;
 }
 
+// Normal field of layer tnum as edge DOFs (float32 tangential value * len/2) on the anomalous
+// edges, sources outermost (LoadE0FromRZ, A2/time_approx.cpp:1263-1352). The layers of the
+// last step stay cached (normal_term drops the others): the stencils reuse them next step.
+const std::vector<double>& CalcCore::normal_layer(int tnum) {
+    auto it = e0_layers_.find(tnum);
+    if (it != e0_layers_.end()) return it->second;
+    const size_t cnt = size_t(M_.n_anom) * npls_;
+    std::vector<float> raw(cnt);
+    load_normal_field(M_, tnum, raw.data());
+    std::vector<double> v(cnt);
+    for (int r = 0; r < npls_; ++r)
+        for (int k = 0; k < M_.n_anom; ++k) {
+            const size_t q = size_t(r) * M_.n_anom + k;
+            v[q] = double(raw[q]) * anom_half_[k];
+        }
+    return e0_layers_.emplace(tnum, std::move(v)).first->second;
+}
+
+// E_n of step it in primary/secondary mode (null otherwise), uploaded into en_. A: the normal
+// field of the step's layer (A2/time_approx.cpp:1013-1016). E: -dE0/dt, two-point between it
+// and its predecessor tn1 (E2/time_approx.cpp:1419-1432, and the first two layers of the
+// 3-point scheme), from layer 3 on in the 3-point scheme minus the derivative at t_it of the
+// parabola through (tn2, tn1, it) (E3/time_approx.cpp:1400-1465).
+const double* CalcCore::normal_term(int it, int tn1, int tn2) {
+    if (!ps_) return nullptr;
+    const std::vector<double>& t = M_.times;
+    const size_t cnt = en_vals_.n;
+    const int s = en_slot_;
+    en_slot_ ^= 1;
+    if (en_ev_used_[s]) EMTD_HIP(hipEventSynchronize(en_ev_[s]));
+    double* h = en_stage_[s];
+    if (form_ == EMTD_FORM_A) {
+        const std::vector<double>& e = normal_layer(it);
+        std::copy(e.begin(), e.end(), h);
+    } else if (scheme_ == 2 || it <= 2) {
+        const std::vector<double>& e0 = normal_layer(it);
+        const std::vector<double>& e1 = normal_layer(tn1);
+        const double dt = t[it] - t[tn1];
+        for (size_t q = 0; q < cnt; ++q) h[q] = -(e0[q] - e1[q]) / dt;
+    } else {
+        const std::vector<double>& e0 = normal_layer(it);
+        const std::vector<double>& e1 = normal_layer(tn1);
+        const std::vector<double>& e2 = normal_layer(tn2);
+        const double tj = t[it], tj1 = t[tn1], tj2 = t[tn2];
+        const double dt = tj - tj2, dt0 = tj - tj1, dt1 = tj1 - tj2, tt = tj;
+        const double c2 = (2.0 * tt - tj - tj1) / (dt1 * dt), c1 = (2.0 * tt - tj - tj2) / (dt1 * dt0),
+                     c0 = (2.0 * tt - tj1 - tj2) / (dt * dt0);
+        for (size_t q = 0; q < cnt; ++q) h[q] = -(e2[q] * c2 - e1[q] * c1 + e0[q] * c0);
+    }
+    EMTD_HIP(hipMemcpyAsync(en_vals_.p, h, cnt * sizeof(double), hipMemcpyHostToDevice, stream_));
+    EMTD_HIP(hipEventRecord(en_ev_[s], stream_));
+    en_ev_used_[s] = true;
+    launch_scatter_rows(stream_, M_.n_anom, n_, npls_, anom_idx_.p, en_vals_.p, en_.p);
+    for (auto q = e0_layers_.begin(); q != e0_layers_.end();) {
+        if (q->first != it && q->first != tn1 && q->first != tn2) q = e0_layers_.erase(q);
+        else ++q;
+    }
+    return en_.p;
+}
+
 void CalcCore::output_layer(int layer, const double* u) {
     if (layer < first_ || layer > last_ || rec_rows_ == 0) return;
     double* out = rec_vals_.p + size_t(layer - first_) * npls_ * rec_rows_;
@@ -471,8 +592,9 @@ void CalcCore::two_layer_step() {
     const int it = L_.two_layer;
     const double dtl = t[1] - t[0];
     if (it == 1) factorize(1.0 / dtl);
+    const double* en = normal_term(it, it - 1, it - 2);
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj, L_.uj, 1.0 / dtl, 0.0,
-               src_.n ? src_.p : nullptr, source_scale(it, dtl), rhs_.p);
+               src_.n ? src_.p : nullptr, source_scale(it, dtl), cdv_.p, en, rhs_.p);
     solve(rhs_.p, L_.uj2, L_.uj);
     std::swap(L_.uj, L_.uj2);
     if (it >= D.nlst) keep_layer(it, L_.uj);
@@ -567,8 +689,9 @@ bool CalcCore::step() {
         L_.mult2 = -L_.dt0 / (L_.dt * L_.dt1);
     }
     if (!L_.fcalc) factorize(L_.mult);
+    const double* en = normal_term(it, L_.tn1, L_.tn2);
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj1, L_.uj2, L_.mult1, L_.mult2,
-               src_.n ? src_.p : nullptr, source_scale(it, -1.0), rhs_.p);
+               src_.n ? src_.p : nullptr, source_scale(it, -1.0), cdv_.p, en, rhs_.p);
     solve(rhs_.p, L_.uj, L_.uj1);
     if (it >= D.nlst) keep_layer(it, L_.uj);
     if (L_.tn1 < last_) output_layer(L_.tn1, L_.uj1);

@@ -62,6 +62,8 @@ class CalcCore {
     void refine(const double* b, double* x, const double* predictor);
     void build_residual_operator();
     double source_scale(int itime, double dt_two_layer) const;
+    const double* normal_term(int it, int tn1, int tn2);
+    const std::vector<double>& normal_layer(int tnum);
     void output_layer(int layer, const double* u);
     double* keep_layer(int layer, const double* u);
 
@@ -84,6 +86,19 @@ class CalcCore {
     DevBuf<int> cci_;
     DevBuf<double> ccv_;
     DevBuf<double> u0_, src_;
+    // primary/secondary mode: C_dsig values on C's pattern, the dense E_n vector (permuted),
+    // the anomalous edges' permuted indices and len/2, loaded normal-field layers, and a
+    // two-slot pinned staging ring for the per-step upload
+    bool ps_ = false;
+    DevBuf<double> cdv_, en_, en_vals_;
+    DevBuf<int> anom_idx_;
+    std::vector<int> anom_edges_;
+    std::vector<double> anom_half_;
+    std::map<int, std::vector<double>> e0_layers_;
+    double* en_stage_[2] = {nullptr, nullptr};
+    hipEvent_t en_ev_[2] = {nullptr, nullptr};
+    bool en_ev_used_[2] = {false, false};
+    int en_slot_ = 0;
     SourceInfo src_info_;
     int nrecB_ = 0, nrecE_ = 0, rec_rows_ = 0;
     DevBuf<int64_t> rec_rp_;

@@ -1,6 +1,11 @@
 // Input-file readers of the calculation module (formats of the reference's working directory).
 #include "model_io.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -201,12 +206,45 @@ ModelFiles read_model_files(const std::string& dir, int formulation) {
             M.variant_scheme = sch;
         }
     }
+    {  // primary/secondary mode inputs (A2/time_approx.cpp:350-370, 1270-1278)
+        std::ifstream in(P("mtrsizeforE0"));
+        if (in) {
+            M.anomal.assign(M.n, 0);
+            int k = 0;
+            for (int i = 0; i < M.n && (in >> k); ++i) {
+                M.anomal[i] = k != 0;
+                M.n_anom += k != 0;
+            }
+        }
+        std::ifstream pn(P("PName"));
+        if (pn) pn >> M.pname;
+    }
     const int form = formulation >= 0 ? formulation : (M.variant_form >= 0 ? M.variant_form : 0);
     if (form == 0)
         M.a0 = read_bin<double>(P("a0.dat"), size_t(M.npls) * M.kuzlov * 3);
     else
         M.v3a = read_bin<double>(P("v3a.dat"), size_t(M.npls) * M.kuzlov);
     return M;
+}
+
+void load_normal_field(const ModelFiles& M, int tnum, float* out) {
+    char name[512];
+    std::snprintf(name, sizeof(name), "/m%s.enor.%04d", M.pname.c_str(), tnum);
+    const size_t bytes = size_t(M.n_anom) * M.npls * sizeof(float);
+    if (bytes == 0) return;
+    const int fd = shm_open(name, O_RDONLY, 0);
+    if (fd < 0) throw Error(0x0002, std::string("normal field object ") + (name + 1) + " not found");
+    struct stat st;
+    if (fstat(fd, &st) != 0 || size_t(st.st_size) < bytes) {
+        close(fd);
+        throw Error(0x0002, std::string("normal field object ") + (name + 1) + " holds fewer than " +
+                                std::to_string(bytes) + " bytes");
+    }
+    void* p = mmap(nullptr, bytes, PROT_READ, MAP_SHARED, fd, 0);
+    close(fd);
+    if (p == MAP_FAILED) throw Error(0x0002, std::string("cannot map normal field object ") + (name + 1));
+    std::memcpy(out, p, bytes);
+    munmap(p, bytes);
 }
 
 }  // namespace emtd

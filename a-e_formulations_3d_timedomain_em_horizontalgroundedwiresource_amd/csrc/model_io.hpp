@@ -28,7 +28,16 @@ struct ModelFiles {
     std::vector<double> a0;                  // npls*kuzlov*3 (A-formulation)
     std::vector<double> v3a;                 // npls*kuzlov (E-formulation)
     int variant_form = -1, variant_scheme = -1;  // from "emtd_variant" when present
+    std::vector<char> anomal;                // n: edges carrying the normal field (mtrsizeforE0)
+    int n_anom = 0;
+    std::string pname;                       // PName: prefix of the normal-field objects
 };
+
+// Normal field of time layer tnum in primary/secondary mode (A2/time_approx.cpp:1263-1352):
+// the named memory object "m<PName>.enor.%04d" (here a POSIX shared-memory object) holding
+// n_anom*npls float32 tangential values, sources outermost, anomalous edges in numbering order.
+// Copies them into out; throws EMTD_ERR_NOFILE when the object is missing or too small.
+void load_normal_field(const ModelFiles& M, int tnum, float* out);
 
 // Reads every input of the module from directory dir (A2/vec_prep_data.cpp:83-257, 834-853,
 // 982-1045; A2/time_approx.cpp:243-287; A2/TaskSP.cpp:467-535; A2/give_out_vec_loop.cpp:46-73).

@@ -1,10 +1,12 @@
 // Per-time-step kernels of the implicit scheme (everything except the triangular sweeps).
 //
 // RHS of one step (A2/time_approx.cpp:1018-1081, 3-point: A3/time_approx.cpp:943-945):
-//   b = C (mult1*u_j1 + mult2*u_j2) + scale * s,   b[Dirichlet] = 0
+//   b = C (mult1*u_j1 + mult2*u_j2) + scale * s [+ C_dsig E_n],   b[Dirichlet] = 0
 // with C the sigma-mass matrix in permuted full CSR (only same-direction couplings are
-// stored), s the line-source shape and scale = I(t) (A) or -dI/dt (E). Dirichlet DOFs are the
-// trailing permuted indices [nn, n), so they are simply not written (the solve ignores them).
+// stored), s the line-source shape and scale = I(t) (A) or -dI/dt (E). In primary/secondary
+// mode C_dsig (the (sigma - sigma0) mass matrix, values on the same pattern) multiplies the
+// normal-field term E_n (A2/time_approx.cpp:1018-1034). Dirichlet DOFs are the trailing
+// permuted indices [nn, n), so they are simply not written (the solve ignores them).
 //
 // Receivers (A2/OutputResultant3d.cpp:421-515): each receiver component is a fixed linear
 // functional of the edge field (centre values -> nodal L2 projection -> trilinear
@@ -20,6 +22,7 @@ __global__ void __launch_bounds__(256) k_rhs(int nn, int64_t n, int npls, const 
                                              const int* __restrict__ ci, const double* __restrict__ cv,
                                              const double* __restrict__ u1, const double* __restrict__ u2,
                                              double m1, double m2, const double* __restrict__ src, double scale,
+                                             const double* __restrict__ cdv, const double* __restrict__ en,
                                              double* __restrict__ b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
@@ -36,9 +39,23 @@ __global__ void __launch_bounds__(256) k_rhs(int nn, int64_t n, int npls, const 
         } else {
             for (int64_t k = k0; k < k1; ++k) acc += cv[k] * (a1[ci[k]] * m1);
         }
+        if (en) {
+            const double* e = en + r * n;
+            double accd = 0.0;
+            for (int64_t k = k0; k < k1; ++k) accd += cdv[k] * e[ci[k]];
+            acc = accd + acc;
+        }
         if (src) acc += src[i + r * n] * scale;
         b[i + r * n] = acc;
     }
+}
+
+__global__ void k_scatter_rows(int cnt, int64_t n, int npls, const int* __restrict__ idx,
+                               const double* __restrict__ vals, double* __restrict__ en) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= int64_t(cnt) * npls) return;
+    const int r = int(k / cnt);
+    en[idx[k % cnt] + r * n] = vals[k];
 }
 
 __global__ void k_zero_tail(double* __restrict__ x, int nn, int64_t n, int npls) {
@@ -133,10 +150,18 @@ void launch_axpy_rows(hipStream_t s, int nn, int64_t n, int npls, const double* 
 
 void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
                 const double* u1, const double* u2, double m1, double m2, const double* src, double scale,
-                double* b) {
+                const double* cdv, const double* en, double* b) {
     if (nn <= 0) return;
     hipLaunchKernelGGL(k_rhs, dim3((nn + 255) / 256), dim3(256), 0, s, nn, n, npls, rp, ci, cv, u1, u2, m1, m2, src,
-                       scale, b);
+                       scale, cdv, en, b);
+}
+
+void launch_scatter_rows(hipStream_t s, int cnt, int64_t n, int npls, const int* idx, const double* vals,
+                         double* en) {
+    const int64_t tot = int64_t(cnt) * npls;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_scatter_rows, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, cnt, n, npls, idx, vals,
+                       en);
 }
 
 void launch_zero_tail(hipStream_t s, double* x, int nn, int64_t n, int npls) {

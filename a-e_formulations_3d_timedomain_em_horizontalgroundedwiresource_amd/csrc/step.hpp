@@ -7,9 +7,14 @@
 
 namespace emtd {
 
+// b = C (m1 u1 + m2 u2) + scale * src [+ Cd en], rows [0, nn); cdv (values of Cd on C's
+// pattern) and en may be null.
 void launch_rhs(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* cv,
                 const double* u1, const double* u2, double m1, double m2, const double* src, double scale,
-                double* b);
+                const double* cdv, const double* en, double* b);
+// en[idx[k] + r*n] = vals[k + r*cnt] for k < cnt, r < npls.
+void launch_scatter_rows(hipStream_t s, int cnt, int64_t n, int npls, const int* idx, const double* vals,
+                         double* en);
 void launch_residual(hipStream_t s, int nn, int64_t n, int npls, const int64_t* rp, const int* ci, const double* vb,
                      const double* vc, const double* db, const double* dc, double mult, const double* b,
                      const double* x, double* r, double* norms);

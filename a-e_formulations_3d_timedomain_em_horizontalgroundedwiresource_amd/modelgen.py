@@ -82,6 +82,7 @@ class ModelSpec:
     n_neib: int = 3
     n_srs_razb: int = 500
     print_window: tuple | None = None                    # (left, right) seconds
+    primary_secondary: str | None = None                 # PName: anomalous field driven by a normal field
 
 
 def time_grid(t_end: float, steps_per_decade: int, ndec: int) -> np.ndarray:
@@ -315,13 +316,28 @@ def write_model(spec: ModelSpec, out: str) -> dict:
     left, right = spec.print_window if spec.print_window else (tq[1], tq[-1])
     with open(os.path.join(out, "timeintervalforprint"), "w") as f:
         f.write(f"{left:.17g} {right:.17g}\n")
-    for name, val in (("fdirect", 1), ("n_neib", spec.n_neib), ("nSrsRazb", spec.n_srs_razb),
+    ps = spec.primary_secondary is not None
+    for name, val in (("fdirect", 0 if ps else 1), ("n_neib", spec.n_neib), ("nSrsRazb", spec.n_srs_razb),
                       ("nthreads.txt", 1)):
         with open(os.path.join(out, name), "w") as f:
             f.write(f"{val}\n")
+    n_anom = 0
+    if ps:
+        anom = anomalous_edges(xyz, edges, ed, mat - 1, sig, sig0)
+        n_anom = int(anom.sum())
+        with open(os.path.join(out, "mtrsizeforE0"), "w") as f:
+            f.write("".join("1\n" if a else "0\n" for a in anom))
+        with open(os.path.join(out, "PName"), "w") as f:
+            f.write(f"{spec.primary_secondary}\n")
 
     hmin = float(min(np.diff(xs).min(), np.diff(ys).min(), np.diff(zs).min()))
-    if spec.formulation == "A":
+    if ps:
+        # the anomalous field starts from rest: zero initial field
+        if spec.formulation == "A":
+            _w_f64(os.path.join(out, "a0.dat"), np.zeros(kuz * 3 * npls))
+        else:
+            _w_f64(os.path.join(out, "v3a.dat"), np.zeros(kuz * npls))
+    elif spec.formulation == "A":
         a0 = np.concatenate([wire_vector_potential(xyz, w[0], w[1], rmin=0.5 * hmin).ravel() for w in wires])
         _w_f64(os.path.join(out, "a0.dat"), a0)
     else:
@@ -330,7 +346,77 @@ def write_model(spec: ModelSpec, out: str) -> dict:
         _w_f64(os.path.join(out, "v3a.dat"), v)
     with open(os.path.join(out, "emtd_variant"), "w") as f:
         f.write(f"{spec.formulation} {spec.scheme}\n")
-    return dict(n=n, kuzlov=kuz, kpar=kpar, nt=len(t), nmat=nmat, npls=npls)
+    return dict(n=n, kuzlov=kuz, kpar=kpar, nt=len(t), nmat=nmat, npls=npls, n_anom=n_anom)
+
+
+# ----------------------------------------------------------------------------- primary/secondary
+SHM_DIR = "/dev/shm"   # POSIX shared-memory objects (shm_open) live here
+
+
+def anomalous_edges(xyz, edges, ed, mat0, sig, sig0, eps: float = 1e-6, z_top: float = 0.0) -> np.ndarray:
+    """Edges that carry the normal field, as the anomalous-field unloader marks them
+    (``M/UnloadAnomalNonstat/time_approx.cpp:160-195``): the 12 edges of every element whose
+    conductivity differs from the normal one by more than ``eps``, minus the edges with a node
+    above the surface (z > z_top + 1e-3)."""
+    sig = np.asarray(sig)
+    sig0 = np.asarray(sig0)
+    elem = np.abs(sig0[mat0] - sig[mat0]) > eps
+    anom = np.zeros(edges.shape[0], dtype=bool)
+    anom[ed[elem].reshape(-1)] = True
+    z0 = z_top + 1e-3
+    above = (xyz[edges[:, 0], 2] > z0) | (xyz[edges[:, 1], 2] > z0)
+    return anom & ~above
+
+
+def normal_field_name(pname: str, tnum: int) -> str:
+    """Object name of the normal field of time layer ``tnum`` (A2/time_approx.cpp:1280-1283)."""
+    return "m%s.enor.%04d" % (pname, tnum)
+
+
+def write_normal_field(pname: str, tnum: int, values) -> str:
+    """Publishes one layer's normal field: float32 tangential values, sources outermost, the
+    anomalous edges in numbering order. Returns the object's path (remove it when done)."""
+    path = os.path.join(SHM_DIR, normal_field_name(pname, tnum))
+    np.ascontiguousarray(values, dtype="<f4").tofile(path)
+    return path
+
+
+def remove_normal_fields(pname: str, ntime: int) -> None:
+    for t in range(ntime):
+        try:
+            os.remove(os.path.join(SHM_DIR, normal_field_name(pname, t)))
+        except OSError:
+            pass
+
+
+def synthetic_normal_fields(spec: ModelSpec, out: str) -> list:
+    """Writes a smooth, decaying synthetic normal field for every time layer of a
+    primary/secondary model written to ``out`` (E0 = exp(-t/tau) * (1, 0.3, 0.1) * g(x, y, z),
+    g a Gaussian around the wire centre; source k scaled by 1/(k+1)); returns the paths."""
+    xs, ys, zs = spec.xs, spec.ys, spec.zs
+    nxn, nyn, nzn = len(xs), len(ys), len(zs)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    xyz = np.stack([X.transpose(2, 1, 0).reshape(-1), Y.transpose(2, 1, 0).reshape(-1),
+                    Z.transpose(2, 1, 0).reshape(-1)], axis=1)
+    edges, _, _ = grid_topology(nxn, nyn, nzn)
+    with open(os.path.join(out, "mtrsizeforE0")) as f:
+        anom = np.array([int(v) for v in f.read().split()[:edges.shape[0]]]) != 0
+    e = edges[anom]
+    axis = xyz[e[:, 1]] - xyz[e[:, 0]]
+    tang = axis / np.linalg.norm(axis, axis=1)[:, None]
+    mid = 0.5 * (xyz[e[:, 0]] + xyz[e[:, 1]])
+    ext = float(min(xs[-1], ys[-1]))
+    g = np.exp(-((mid[:, 0] / ext) ** 2 + (mid[:, 1] / ext) ** 2 + (mid[:, 2] / ext) ** 2))
+    dirv = np.array([1.0, 0.3, 0.1])
+    base = g * (tang @ dirv)
+    t = spec.times
+    tau = 0.2 * float(t[-1])
+    npls = 1 + len(spec.extra_wires)
+    paths = []
+    for it in range(len(t)):
+        vals = np.concatenate([base * np.exp(-t[it] / tau) / (k + 1) for k in range(npls)])
+        paths.append(write_normal_field(spec.primary_secondary, it, vals))
+    return paths
 
 
 def default_receivers(y: float = 300.0, n: int = 21, half: float = 1000.0) -> np.ndarray:

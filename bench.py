@@ -8,7 +8,10 @@ RHS (mass-matrix SpMV + source), forward/backward triangular sweeps with the GPU
 receiver evaluation; the first step of each decade also refactorizes the system matrix.
 
 Prints ONE JSON line. ``value`` = timed steps / wall time of the timed region (inputs and
-factor storage already resident in HBM). Two per-step hot paths are measured with HIP events
+factor storage already resident in HBM) of the FP64 path. Beside it, the same model runs with
+the FP32 factor + FP64 iterative refinement path (``fp32_ir``: its steps/s, speedup over FP64,
+refinement iterations per step, FP64 fallbacks, final residual, and the largest receiver
+difference from the FP64 run relative to the FP64 values' largest magnitude). Two per-step hot paths are measured with HIP events
 on the solver's stream: one triangular solve (algorithmic bytes = 8 bytes x nnz(L) read by each
 of the two sweeps, per right-hand side; HBM roofline) and one refactorization (flops = sum over
 fronts of p^3/3 + p^2 u + p u^2; FP64 MFMA roofline). ``roofline`` is the one with the larger
@@ -25,6 +28,8 @@ import shutil
 import sys
 import tempfile
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -155,6 +160,88 @@ def measured_traffic(args, cells, what):
     return None
 
 
+def timed_leg(emtd, d, spec, precision, args):
+    """Runs the whole time loop of the model in ``d`` at one precision: ``warmup`` untimed
+    steps, then ``steps`` timed steps. Returns the JSON line's fields and the receiver values."""
+    import numpy as np
+    prec = emtd.FP64 if precision == "fp64" else emtd.FP32_IR
+    cells = tuple(int(v) for v in args.cells.split("x")) if args.cells else None
+    t0 = time.time()
+    c = emtd.Calc(d, 0 if spec.formulation == "A" else 1, spec.scheme, prec)
+    ci = c.info()
+    log(f"[bench] {precision} setup {time.time() - t0:.1f} s: n={ci.n_edges} factored={ci.n_factored} "
+        f"fronts={ci.n_fronts} levels={ci.n_levels} nnzL={ci.nnz_l:.4g} flops={ci.factor_flops:.4g}")
+    total_steps = ci.ntime - 1
+    W = min(args.warmup, total_steps - 1)
+    K = min(args.steps, total_steps - W)
+    log(f"[bench] {precision} warmup {W} steps")
+    c.step(W)
+    inf0 = c.info()
+    c.set_timing(True)
+    log(f"[bench] {precision} timed {K} steps")
+    t1 = time.perf_counter()
+    got = c.step(K)
+    t2 = time.perf_counter()
+    c.set_timing(False)
+    inf1 = c.info()
+    assert got == K, (got, K)
+    c.step(-1)
+    receivers = {"B": c.receiver_values(0), "E": c.receiver_values(1)}
+    elapsed = t2 - t1
+    value = K / elapsed
+    nsolve = inf1.timed_solves
+    solve_s = inf1.solve_gpu_seconds / max(nsolve, 1)
+    elem = 8 if precision == "fp64" else 4
+    bytes_solve = 2 * elem * ci.nnz_l * ci.npls
+    achieved = bytes_solve / solve_s / 1e9
+    nfac = inf1.factorizations - inf0.factorizations
+    fac_gpu = (inf1.factor_gpu_seconds - inf0.factor_gpu_seconds) / max(nfac, 1)
+    fac_tflops = ci.factor_flops / fac_gpu / 1e12 if nfac else None
+    log(f"[bench] {precision}: {K} steps in {elapsed:.3f} s -> {value:.2f} steps/s; solve {solve_s * 1e3:.3f} ms "
+        f"({achieved:.0f} GB/s); {nfac} factorizations {fac_gpu * 1e3:.1f} ms each ({fac_tflops} TF/s)")
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "timesteps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(1e3 * elapsed / K, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"config {args.config}: {spec.formulation}-formulation {spec.scheme}-point, "
+                                f"{len(spec.xs) - 1}x{len(spec.ys) - 1}x{len(spec.zs) - 1} cells, "
+                                f"{ci.n_edges} edge DOFs, {total_steps} steps in {ci.n_decades} decades"),
+                   "n_edges": ci.n_edges, "nnz_L": ci.nnz_l, "factor_flops": ci.factor_flops,
+                   "precision": precision, "npls": ci.npls,
+                   "timed_factorizations": nfac, "factor_ms": round(fac_gpu * 1e3, 3),
+                   "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
+                   "setup_s": round(ci.setup_seconds, 2),
+                   "refine_iterations_per_step": round((inf1.refine_iterations - inf0.refine_iterations) / K, 4),
+                   "fallbacks": int(inf1.fallbacks), "final_relative_residual": inf1.last_residual},
+        "roofline": None,
+    }
+    # the two per-step hot paths; "roofline" is the one with the larger share of the timed
+    # region (the other is reported beside it)
+    peak_tf = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+    prof = argparse.Namespace(config=args.config, precision=precision)
+    r_solve = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(prof, cells, "solve"),
+               "kernel": ("one triangular solve (forward + backward sweep kernels); bytes = 2 sweeps x %d B "
+                          "x nnz(L) x sources" % elem),
+               "bytes_per_launch": bytes_solve, "launch_ms": round(solve_s * 1e3, 4),
+               "share_of_timed_region": round(nsolve * solve_s / elapsed, 4)}
+    r_fact = None
+    if nfac:
+        r_fact = {"bound": "mfma", "achieved": round(fac_tflops, 3), "peak": peak_tf, "unit": "TFLOP/s",
+                  "frac": round(fac_tflops / peak_tf, 4), "traffic": measured_traffic(prof, cells, "factor"),
+                  "kernel": ("one multifrontal factorization (panel, update, extend-add kernels); flops = "
+                             "sum over fronts of p^3/3 + p^2 u + p u^2"),
+                  "flops_per_launch": ci.factor_flops, "launch_ms": round(fac_gpu * 1e3, 3),
+                  "share_of_timed_region": round(nfac * fac_gpu / elapsed, 4)}
+    if r_fact and r_fact["share_of_timed_region"] > r_solve["share_of_timed_region"]:
+        out["roofline"], out["roofline_other"] = r_fact, r_solve
+    else:
+        out["roofline"], out["roofline_other"] = r_solve, r_fact
+    c.close()
+    return {"out": out, "receivers": receivers}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +251,8 @@ def main():
     ap.add_argument("--cells", type=str, default=None, help="override cells, e.g. 40x40x30")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip the FP32+IR leg that runs beside the FP64 one on the same model")
     ap.add_argument("--no-serving", action="store_true", help="skip the short serving sample")
     ap.add_argument("--max-active", type=int, default=6, help="concurrent requests (configs 4/5)")
     ap.add_argument("--requests", type=int, default=40, help="requests of the config-5 run")
@@ -211,81 +300,32 @@ def main():
     t0 = time.time()
     d, spec, info = make_workload(args.config, cells)
     log(f"[bench] model: {info} in {time.time() - t0:.1f} s")
-    prec = emtd.FP64 if args.precision == "fp64" else emtd.FP32_IR
     try:
-        t0 = time.time()
-        c = emtd.Calc(d, 0 if spec.formulation == "A" else 1, spec.scheme, prec)
-        ci = c.info()
-        log(f"[bench] setup {time.time() - t0:.1f} s: n={ci.n_edges} factored={ci.n_factored} "
-            f"fronts={ci.n_fronts} levels={ci.n_levels} nnzL={ci.nnz_l:.4g} flops={ci.factor_flops:.4g}")
-        total_steps = ci.ntime - 1
-        W = min(args.warmup, total_steps - 1)
-        K = min(args.steps, total_steps - W)
-        log(f"[bench] warmup {W} steps")
-        c.step(W)
-        inf0 = c.info()
-        c.set_timing(True)
-        log(f"[bench] timed {K} steps")
-        t1 = time.perf_counter()
-        got = c.step(K)
-        t2 = time.perf_counter()
-        c.set_timing(False)
-        inf1 = c.info()
-        assert got == K, (got, K)
-        elapsed = t2 - t1
-        value = K / elapsed
-        nsolve = inf1.timed_solves
-        solve_s = inf1.solve_gpu_seconds / max(nsolve, 1)
-        elem = 8 if args.precision == "fp64" else 4
-        bytes_solve = 2 * elem * ci.nnz_l * ci.npls
-        achieved = bytes_solve / solve_s / 1e9
-        nfac = inf1.factorizations - inf0.factorizations
-        fac_gpu = (inf1.factor_gpu_seconds - inf0.factor_gpu_seconds) / max(nfac, 1)
-        fac_tflops = ci.factor_flops / fac_gpu / 1e12 if nfac else None
-        log(f"[bench] {K} steps in {elapsed:.3f} s -> {value:.2f} steps/s; solve {solve_s * 1e3:.3f} ms "
-            f"({achieved:.0f} GB/s); {nfac} factorizations {fac_gpu * 1e3:.1f} ms each "
-            f"({fac_tflops} TF/s)")
-        out = {
-            "metric": METRIC, "value": round(value, 4), "unit": "timesteps/s", "n_gpus": 1, "steps": K,
-            "warmup": W, "ms_per_step": round(1e3 * elapsed / K, 4), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-            "data": "synthetic",
-            "config": {"workload": (f"config {args.config}: {spec.formulation}-formulation {spec.scheme}-point, "
-                                    f"{len(spec.xs) - 1}x{len(spec.ys) - 1}x{len(spec.zs) - 1} cells, "
-                                    f"{ci.n_edges} edge DOFs, {total_steps} steps in {ci.n_decades} decades"),
-                       "n_edges": ci.n_edges, "nnz_L": ci.nnz_l, "factor_flops": ci.factor_flops,
-                       "precision": args.precision, "npls": ci.npls,
-                       "timed_factorizations": nfac, "factor_ms": round(fac_gpu * 1e3, 3),
-                       "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
-                       "setup_s": round(ci.setup_seconds, 2),
-                       "refine_iterations_per_step": round((inf1.refine_iterations - inf0.refine_iterations) / K, 4),
-                       "fallbacks": int(inf1.fallbacks), "final_relative_residual": inf1.last_residual},
-            "roofline": None,
-        }
-        # the two per-step hot paths; "roofline" is the one with the larger share of the timed
-        # region (the other is reported beside it)
-        peak_tf = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
-        r_solve = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args, cells, "solve"),
-                   "kernel": ("one triangular solve (forward + backward sweep kernels); bytes = 2 sweeps x %d B "
-                              "x nnz(L) x sources" % elem),
-                   "bytes_per_launch": bytes_solve, "launch_ms": round(solve_s * 1e3, 4),
-                   "share_of_timed_region": round(nsolve * solve_s / elapsed, 4)}
-        r_fact = None
-        if nfac:
-            r_fact = {"bound": "mfma", "achieved": round(fac_tflops, 3), "peak": peak_tf, "unit": "TFLOP/s",
-                      "frac": round(fac_tflops / peak_tf, 4), "traffic": measured_traffic(args, cells, "factor"),
-                      "kernel": ("one multifrontal factorization (panel, update, extend-add kernels); flops = "
-                                 "sum over fronts of p^3/3 + p^2 u + p u^2"),
-                      "flops_per_launch": ci.factor_flops, "launch_ms": round(fac_gpu * 1e3, 3),
-                      "share_of_timed_region": round(nfac * fac_gpu / elapsed, 4)}
-        if r_fact and r_fact["share_of_timed_region"] > r_solve["share_of_timed_region"]:
-            out["roofline"], out["roofline_other"] = r_fact, r_solve
-        else:
-            out["roofline"], out["roofline_other"] = r_solve, r_fact
-        c.close()
+        legs = [args.precision]
+        if args.compare and args.precision == "fp64":
+            legs.append("fp32")
+        res = {p: timed_leg(emtd, d, spec, p, args) for p in legs}
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    main_leg = res[args.precision]
+    out = main_leg["out"]
+    if "fp32" in res and args.precision == "fp64":
+        f32, f64 = res["fp32"], res["fp64"]
+        errs = {}
+        for kind in ("B", "E"):
+            a, b = f32["receivers"][kind], f64["receivers"][kind]
+            scale = float(np.abs(b).max()) if b.size else 0.0
+            errs[kind] = float(np.abs(a - b).max() / scale) if scale > 0 else None
+        out["fp32_ir"] = {
+            "timesteps_per_s": f32["out"]["value"], "ms_per_step": f32["out"]["ms_per_step"],
+            "speedup_vs_fp64": round(f32["out"]["value"] / f64["out"]["value"], 4),
+            "max_rel_receiver_err_vs_fp64": errs,
+            "meets_1e-5": all(e is not None and e <= 1e-5 for e in errs.values()),
+            **{k: f32["out"]["config"][k] for k in ("timed_factorizations", "factor_ms", "factor_tflops",
+                                                    "solve_ms", "refine_iterations_per_step", "fallbacks",
+                                                    "final_relative_residual")},
+            "roofline": f32["out"]["roofline"]}
+        log(f"[bench] FP32+IR: {out['fp32_ir']}")
     if not args.no_serving and args.config == 3 and cells is None:
         # completion latency under load: 16 requests of the serving generator (24-48 cells per
         # axis, 150 steps each) arriving as a Poisson stream at 4/s

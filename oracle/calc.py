@@ -19,6 +19,9 @@ Follows:
   ``A2/Subdomain.cpp:418-441``, interpolation ``A2/OutputResultant3d.cpp:421-515`` with
   coefficients of ``A2/T_Brick.cpp:1792-1805``; output layer bookkeeping
   ``A2/time_approx.cpp:1116-1128``;
+* primary/secondary mode (``fdirect`` = 0 with ``mtrsizeforE0``): RHS += C_dsig E_n,
+  ``A2/time_approx.cpp:350-370, 441-470, 1013-1034, 1263-1352``, ``E2/time_approx.cpp:1419-1432``,
+  ``E3/time_approx.cpp:1400-1465``;
 * time differentiation and tables: ``A2/give_out_vec_loop.cpp:240-615``, 3-point variant
   ``A3/give_out_vec_loop.cpp:313-367``, E-formulation ``E2/give_out_vec_loop.cpp:255-312``.
 
@@ -30,7 +33,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from . import fem
+from . import fem, refio
 from .refio import Model
 
 
@@ -192,6 +195,28 @@ def source_scale(model: Model, form: str, itime: int, dt_two_layer: float | None
     else:
         djdt = (cur[itime] - cur[itime - 1]) / dt_back
     return -djdt
+
+
+# ----------------------------------------------------------------------------- normal field
+def normal_term(model: Model, form: str, scheme: int, it: int, tn1: int, tn2: int, load) -> np.ndarray:
+    """E_n of step ``it`` in primary/secondary mode, (npls, n) edge DOFs; ``load(t)`` returns
+    the normal field of layer t (``refio.read_normal_field``).
+
+    A-formulation: the normal field of the step's own layer (``A2/time_approx.cpp:1013-1016``,
+    ``A3/time_approx.cpp:981``). E-formulation: -dE0/dt, two-point between ``it`` and its
+    predecessor ``tn1`` (``E2/time_approx.cpp:1419-1432``; also the 3-point scheme's first two
+    layers), and for the 3-point scheme from layer 3 on minus the derivative at t_it of the
+    parabola through (tn2, tn1, it) (``E3/time_approx.cpp:1400-1465``)."""
+    t = model.times
+    if form == "A":
+        return load(it)
+    if scheme == 2 or it <= 2:
+        return -(load(it) - load(tn1)) / (t[it] - t[tn1])
+    tj, tj1, tj2 = t[it], t[tn1], t[tn2]
+    dt, dt0, dt1 = tj - tj2, tj - tj1, tj1 - tj2
+    tt = tj
+    return -(load(tn2) * (2.0 * tt - tj - tj1) / (dt1 * dt) - load(tn1) * (2.0 * tt - tj - tj2) / (dt1 * dt0)
+             + load(it) * (2.0 * tt - tj1 - tj2) / (dt * dt0))
 
 
 # ----------------------------------------------------------------------------- receivers
@@ -370,6 +395,22 @@ def run(model: Model, form: str, scheme: int, solver=None, keep_fields=False) ->
     bnd = fem.boundary_edges(model)
     u0 = initial_state(model, form)
     src = source_vectors(model) if model.fdirect else None
+    # primary/secondary mode: RHS += C_dsig E_n with C_dsig the (sigma - sigma0) mass matrix
+    # (A2/time_approx.cpp:350-370, 441-470, 1013-1034)
+    ps = model.anomal is not None and bool(model.anomal.any()) and not model.fdirect
+    Cd = None
+    en_cache = {}
+    if ps:
+        _, _, cd_ggl, cd_di = fem.assemble(model, ig, jg, sigma=model.sigma - model.sigma0)
+        Cd = fem.rsf_to_csc(ig, jg, cd_ggl, cd_di).tocsr()
+
+    def load_e0(layer):
+        if layer not in en_cache:
+            en_cache[layer] = refio.read_normal_field(model, layer)
+        return en_cache[layer]
+
+    def en_of(it, tn1, tn2):
+        return normal_term(model, form, scheme, it, tn1, tn2, load_e0) if ps else None
     first, last = print_window(model)
     recB = ReceiverSet(model, model.recB, "B")
     recE = ReceiverSet(model, model.recE, "E")
@@ -388,9 +429,11 @@ def run(model: Model, form: str, scheme: int, solver=None, keep_fields=False) ->
 
     Cfull = fem.rsf_to_csc(ig, jg, c_ggl, c_di).tocsr()
 
-    def rhs(u1, u2, m1, m2, scale):
+    def rhs(u1, u2, m1, m2, scale, en=None):
         x = u1 * m1 + (u2 * m2 if m2 != 0.0 else 0.0)
         pr = (Cfull @ x.T).T
+        if en is not None:
+            pr = pr + (Cd @ en.T).T
         if scale != 0.0 and src is not None:
             pr = pr + src * scale
         pr[:, bnd] = 0.0
@@ -419,7 +462,7 @@ def run(model: Model, form: str, scheme: int, solver=None, keep_fields=False) ->
                 solve2 = factorize(1.0 / dtl)
                 for it in (1, 2):
                     sc = source_scale(model, form, it, dtl) if src is not None else 0.0
-                    pr = rhs(u_j, u_j, 1.0 / dtl, 0.0, sc)
+                    pr = rhs(u_j, u_j, 1.0 / dtl, 0.0, sc, en_of(it, it - 1, it - 2))
                     u_j = np.stack([solve2(pr[ip]) for ip in range(model.npls)])
                     if it >= nlst:
                         stored[it] = u_j.copy()
@@ -468,7 +511,7 @@ def run(model: Model, form: str, scheme: int, solver=None, keep_fields=False) ->
             if not fcalc:
                 solve = factorize(mult)
             sc = source_scale(model, form, it, None) if src is not None else 0.0
-            pr = rhs(u_j1, u_j2, mult1, mult2, sc)
+            pr = rhs(u_j1, u_j2, mult1, mult2, sc, en_of(it, tn1, tn2))
             u_j = np.stack([solve(pr[ip]) for ip in range(model.npls)])
             if it >= nlst:
                 stored[it] = u_j.copy()

@@ -138,13 +138,18 @@ def portrait(ed: np.ndarray, n: int):
     return ig, cols.astype(np.int64)
 
 
-def assemble(model, ig, jg):
-    """B (curl-curl / mu) and C (sigma mass) in lower RSF: (b_ggl, b_di, c_ggl, c_di)."""
+def assemble(model, ig, jg, sigma=None):
+    """B (curl-curl / mu) and C (sigma mass) in lower RSF: (b_ggl, b_di, c_ggl, c_di).
+
+    ``sigma`` (per material) replaces the model's conductivity in C; sigma - sigma0 gives the
+    mass matrix of the primary/secondary term (``T_Brick`` case 4, ``A2/T_Brick.cpp:518-526``).
+    """
     n = len(ig) - 1
     h = element_sizes(model)
     B, C = local_matrices(h)
     B = B / model.mu[model.nvkat][:, None, None]
-    C = C * model.sigma[model.nvkat][:, None, None]
+    sig = model.sigma if sigma is None else np.asarray(sigma, float)
+    C = C * sig[model.nvkat][:, None, None]
     out = []
     for Mloc in (B, C):
         rows = np.repeat(model.ed, 12, axis=1).reshape(-1)

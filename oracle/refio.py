@@ -3,8 +3,9 @@
 Follows ``A2/vec_prep_data.cpp:83-257`` (materials, receivers, mesh), ``A2/in_out.cpp:222-294``
 (``inftry.dat``), ``A2/vec_prep_data.cpp:834-853`` (``infite.0``), ``A2/time_approx.cpp:243-287``
 (edges), ``A2/TaskSP.cpp:467-535`` (group / receiver counts),
-``A2/give_out_vec_loop.cpp:46-73`` (print window) and ``A2/vec_prep_data.cpp:982-1045``
-(current function, line source).
+``A2/give_out_vec_loop.cpp:46-73`` (print window), ``A2/vec_prep_data.cpp:982-1045``
+(current function, line source) and the primary/secondary input of ``A2/time_approx.cpp:350-370,
+1263-1352`` (``mtrsizeforE0``, ``PName`` and the per-layer normal-field objects).
 """
 from __future__ import annotations
 
@@ -15,6 +16,10 @@ from dataclasses import dataclass
 import numpy as np
 
 MU_0 = 1.25663706143591729e-6  # A2/vfem_const.h:21
+
+# The normal field of time layer t is the named memory object "m<PName>.enor.%04d"
+# (A2/time_approx.cpp:1280-1287). POSIX shared-memory objects of that name live here.
+SHM_DIR = "/dev/shm"
 
 
 @dataclass
@@ -44,6 +49,8 @@ class Model:
     print_window: tuple
     a0: np.ndarray | None   # (npls, kuzlov, 3)
     v3a: np.ndarray | None  # (npls, kuzlov)
+    anomal: np.ndarray | None = None  # (n,) bool, ``mtrsizeforE0``
+    pname: str | None = None          # ``PName``
 
 
 def _read_flag(path, default, cast=float):
@@ -148,10 +155,51 @@ def read_model(d: str) -> Model:
         a0 = np.fromfile(j("a0.dat"), dtype="<f8", count=3 * kuz * npls).reshape(npls, kuz, 3)
     if os.path.exists(j("v3a.dat")):
         v3a = np.fromfile(j("v3a.dat"), dtype="<f8", count=kuz * npls).reshape(npls, kuz)
+    anomal = pname = None
+    if os.path.exists(j("mtrsizeforE0")):   # A2/time_approx.cpp:355-368: one 0/1 per edge
+        with open(j("mtrsizeforE0")) as f:
+            anomal = np.array([int(v) for v in f.read().split()[:n]], dtype=np.int64) != 0
+    if os.path.exists(j("PName")):
+        with open(j("PName")) as f:
+            tok = f.read().split()
+        pname = tok[0] if tok else None
     return Model(kuz, kpar, xyz, nver, nvkat, sig, sig0, mu, l13d, edges, ed, times, recB, recE,
                  npls, rb, re_, fdirect, current, wires,
                  int(_read_flag(j("nSrsRazb"), 500, int)), int(_read_flag(j("n_neib"), 5, int)),
-                 window, a0, v3a)
+                 window, a0, v3a, anomal, pname)
+
+
+def normal_field_name(pname: str, tnum: int) -> str:
+    """Name of the normal-field object of time layer ``tnum`` (A2/time_approx.cpp:1280-1283)."""
+    return "m%s.enor.%04d" % (pname, tnum)
+
+
+def edge_half_lengths(model: Model) -> np.ndarray:
+    d = model.xyz[model.edges[:, 1]] - model.xyz[model.edges[:, 0]]
+    return 0.5 * np.sqrt((d * d).sum(axis=1))
+
+
+def read_normal_field(model: Model, tnum: int) -> np.ndarray:
+    """``LoadE0FromRZ`` (A2/time_approx.cpp:1263-1352): float32 tangential normal field per
+    anomalous edge, sources outermost, edges in numbering order; returned as edge DOFs
+    (value * len/2, zero on the other edges), shape (npls, n)."""
+    n = model.edges.shape[0]
+    idx = np.nonzero(model.anomal)[0]
+    path = os.path.join(SHM_DIR, normal_field_name(model.pname, tnum))
+    raw = np.fromfile(path, dtype="<f4", count=len(idx) * model.npls)
+    if raw.size != len(idx) * model.npls:
+        raise OSError(f"normal field object {path} holds {raw.size} values, expected {len(idx) * model.npls}")
+    out = np.zeros((model.npls, n))
+    out[:, idx] = raw.reshape(model.npls, len(idx)).astype(np.float64) * edge_half_lengths(model)[idx]
+    return out
+
+
+def write_normal_field(pname: str, tnum: int, values: np.ndarray) -> str:
+    """Writes one layer's normal-field object (float32 values, sources outermost, anomalous
+    edges in numbering order) the way the normal-field producer publishes it; returns its path."""
+    path = os.path.join(SHM_DIR, normal_field_name(pname, tnum))
+    np.ascontiguousarray(values, dtype="<f4").tofile(path)
+    return path
 
 
 def write_tables(d: str, model: Model, t_first: int, t_last: int, eds, bfield, efield, form: str):

@@ -177,3 +177,62 @@ def test_calc_two_sources_matches_oracle(emtd, tmp_path, form, scheme):
         assert a.shape == b.shape, stem
         scale = max(np.abs(b[:, 1:]).max(), 1e-300)
         assert np.abs(a[:, 1:] - b[:, 1:]).max() / scale < TOL64, stem
+
+
+PS_CASES = [("A", 2, True), ("A", 3, False), ("E", 2, False), ("E", 3, True)]
+
+
+@pytest.mark.parametrize("form,scheme,two", PS_CASES)
+def test_calc_primary_secondary_matches_oracle(emtd, mg, tmp_path, form, scheme, two):
+    """Primary/secondary mode (fdirect = 0, mtrsizeforE0, PName and one normal-field object per
+    layer in POSIX shared memory): the anomalous field driven by C_dsig E_n, receivers and the
+    last field against the oracle on the same files at the FP64 bar."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(8, 8, 6), steps=(5, 3), form=form, scheme=scheme)
+    spec.blocks = [mg.Block(-20.0, 15.0, -15.0, 20.0, -28.0, -8.0, 1.5)]
+    spec.layer_tops = [0.0, 12.0]
+    spec.layer_sigma = [0.02, 0.1]
+    if two:
+        spec.extra_wires = [((-10.0, -20.0, 0.0), (25.0, 15.0, 0.0))]
+    spec.primary_secondary = f"emtdgpu{os.getpid()}{form}{scheme}"
+    info = mg.write_model(spec, d)
+    assert info["n_anom"] > 0
+    print("case", form, scheme, "sources", info["npls"], "anomalous edges", info["n_anom"], flush=True)
+    mg.synthetic_normal_fields(spec, d)
+    try:
+        m = refio.read_model(d)
+        ref = calc.run(m, form, scheme, keep_fields=True)
+        c = emtd.Calc(d, 0 if form == "A" else 1, scheme, emtd.FP64)
+        c.run()
+        for kind, want in ((0, ref.valB), (1, ref.valE)):
+            got = c.receiver_values(kind)
+            scale = np.abs(want).max()
+            assert scale > 0
+            err = np.abs(got - want).max() / scale
+            print("kind", kind, "receiver max rel err", err, flush=True)
+            assert err < TOL64
+        last = c.last_field()
+        want = ref.fields[max(ref.fields)]
+        err = np.abs(last - want).max() / np.abs(want).max()
+        print("last field max rel err", err, flush=True)
+        assert err < TOL64
+        c.close()
+    finally:
+        mg.remove_normal_fields(spec.primary_secondary, len(spec.times))
+
+
+def test_calc_primary_secondary_missing_layer_fails(emtd, mg, tmp_path):
+    """A missing normal-field object stops the calculation with EMTD_ERR_NOFILE."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(6, 6, 4), steps=(3, 2))
+    spec.blocks = [mg.Block(-15.0, 15.0, -15.0, 15.0, -20.0, -5.0, 1.0)]
+    spec.primary_secondary = f"emtdgpumiss{os.getpid()}"
+    mg.write_model(spec, d)
+    mg.synthetic_normal_fields(spec, d)
+    os.remove(os.path.join(mg.SHM_DIR, mg.normal_field_name(spec.primary_secondary, 3)))
+    try:
+        c = emtd.Calc(d, 0, 2, emtd.FP64)
+        with pytest.raises(emtd.EmtdError) as ei:
+            c.run()
+        assert ei.value.code == 0x0002
+        c.close()
+    finally:
+        mg.remove_normal_fields(spec.primary_secondary, len(spec.times))

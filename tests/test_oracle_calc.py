@@ -249,3 +249,76 @@ def test_sources_superpose(tmp_path):
             scale = np.abs(b).max()
             assert scale > 0
             assert np.abs(a - b).max() / scale < 1e-10, (k, attr)
+
+
+# ----------------------------------------------------------------------------- primary/secondary
+def _ps_spec(form, scheme, two_sources):
+    mg = modelgen()
+    spec = mg.config_spec(1, small=(8, 8, 6))
+    spec.blocks = [mg.Block(-20.0, 15.0, -15.0, 20.0, -28.0, -8.0, 1.5)]
+    spec.layer_tops = [0.0, 12.0]
+    spec.layer_sigma = [0.02, 0.1]
+    spec.times = mg.time_grid(1e-3, 4, 3)
+    spec.formulation, spec.scheme = form, scheme
+    if two_sources:
+        spec.extra_wires = [((-10.0, -20.0, 0.0), (25.0, 15.0, 0.0))]
+    tq = mg.quantize_times(spec.times)
+    spec.print_window = (tq[2], tq[-1]) if scheme == 3 else (tq[1], tq[-1])
+    return spec
+
+
+@pytest.mark.parametrize("form,scheme,two", [("A", 2, True), ("A", 3, False), ("E", 2, False), ("E", 3, True)])
+def test_primary_secondary_is_total_minus_normal(tmp_path, form, scheme, two):
+    """Superposition pins the primary/secondary path (C_dsig assembly, the normal-field object
+    layout and len/2 scaling, and the E-formulation's -dE0/dt stencils): with the normal field
+    taken from a run on the normal (block-free) model, the anomalous field of the
+    primary/secondary run must equal total minus normal at every layer, up to the float32
+    rounding of the published normal field."""
+    import copy
+    import os
+    mg = modelgen()
+    spec = _ps_spec(form, scheme, two)
+    normal = copy.deepcopy(spec)
+    normal.blocks = []
+    runs = {}
+    for name, sp_ in (("total", spec), ("normal", normal)):
+        d = str(tmp_path / name)
+        mg.write_model(sp_, d)
+        m = refio.read_model(d)
+        runs[name] = (m, calc.run(m, form, scheme, keep_fields=True))
+    m_n, r_n = runs["normal"]
+    u0_n = calc.initial_state(m_n, form)
+    v = dict(r_n.fields)
+    v[0] = u0_n
+    pname = f"emtdtest{os.getpid()}{form}{scheme}"
+    ps = copy.deepcopy(spec)
+    ps.primary_secondary = pname
+    d = str(tmp_path / "ps")
+    info = mg.write_model(ps, d)
+    m_ps = refio.read_model(d)
+    assert info["n_anom"] == int(m_ps.anomal.sum()) > 0
+    idx = np.nonzero(m_ps.anomal)[0]
+    half = refio.edge_half_lengths(m_ps)[idx]
+    nt = len(m_ps.times)
+    try:
+        for it in range(nt):
+            if form == "E":
+                e0 = v[it]
+            elif it == 0:
+                e0 = np.zeros_like(v[0])
+            else:
+                # minus the scheme's own time derivative of the normal vector potential
+                e0 = calc.normal_term(m_ps, "E", scheme, it, int(r_n.vpre1[it]), int(r_n.vpre2[it]), lambda k: v[k])
+            mg.write_normal_field(pname, it, (e0[:, idx] / half).reshape(-1))
+        r_ps = calc.run(m_ps, form, scheme, keep_fields=True)
+    finally:
+        mg.remove_normal_fields(pname, nt)
+    _, r_t = runs["total"]
+    worst = 0.0
+    for it, w in r_ps.fields.items():
+        ref = r_t.fields[it] - v[it]
+        scale = np.abs(ref).max()
+        assert scale > 1e-3 * np.abs(r_t.fields[it]).max()   # the block matters
+        worst = max(worst, np.abs(w - ref).max() / scale)
+    print("worst", worst)
+    assert worst < 2e-5, worst
