@@ -80,7 +80,7 @@ EXPORTS = [
     "emtd_last_error", "emtd_portrait", "emtd_assemble",
     "emtd_solver_create", "emtd_solver_factorize", "emtd_solver_solve_nrhs", "emtd_solver_stop",
     "emtd_solver_destroy", "emtd_solver_get_stats",
-    "emtd_calc_create", "emtd_calc_run", "emtd_calc_step", "emtd_calc_set_timing", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
+    "emtd_calc_create", "emtd_calc_run", "emtd_calc_step", "emtd_calc_set_timing", "emtd_calc_set_snapshots", "emtd_calc_write_outputs", "emtd_calc_receiver_values",
     "emtd_calc_last_field", "emtd_calc_get_info", "emtd_calc_destroy", "emtd_calc_nonstat_3d_line",
     "emtd_server_create", "emtd_server_submit", "emtd_server_wait", "emtd_server_poll", "emtd_server_drain",
     "emtd_server_get_stats", "emtd_server_destroy",
@@ -207,6 +207,10 @@ class Calc:
 
     def set_timing(self, on: bool = True):
         _check(lib().emtd_calc_set_timing(self.h, C.c_int(1 if on else 0)))
+
+    def set_snapshots(self, on: bool = True):
+        """Write v3.<t> for every layer kept for a restart (as the reference module does)."""
+        _check(lib().emtd_calc_set_snapshots(self.h, C.c_int(1 if on else 0)))
 
     def write_outputs(self):
         _check(lib().emtd_calc_write_outputs(self.h))

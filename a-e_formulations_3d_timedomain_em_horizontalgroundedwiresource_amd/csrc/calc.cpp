@@ -177,7 +177,9 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
                 }
                 len = std::sqrt(len);
                 double u;
-                if (form_ == EMTD_FORM_A) {
+                if (!M_.u0_edges.empty()) {
+                    u = M_.u0_edges[size_t(ip) * n_ + i];
+                } else if (form_ == EMTD_FORM_A) {
                     const double* a0 = M_.a0.data() + size_t(ip) * M_.kuzlov * 3;
                     double s = 0;
                     for (int a = 0; a < 3; ++a) s += 0.5 * (a0[3 * v1 + a] + a0[3 * v2 + a]) * axis[a];
@@ -539,7 +541,25 @@ double* CalcCore::keep_layer(int layer, const double* u) {
     auto it = stored_.find(layer);
     if (it == stored_.end()) it = stored_.emplace(layer, DevBuf<double>(size_t(n_) * npls_)).first;
     EMTD_HIP(hipMemcpyAsync(it->second.p, u, size_t(n_) * npls_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (snapshots_) write_snapshot(layer, u);
     return it->second.p;
+}
+
+// v3.<layer>: the layer's edge field, n x npls doubles in original numbering, column-major by
+// source (A2/time_approx.cpp:1093-1104). Synchronous: used by the drop-in process path.
+void CalcCore::write_snapshot(int layer, const double* u) const {
+    const size_t vn = size_t(n_) * npls_;
+    std::vector<double> perm(vn), orig(vn);
+    EMTD_HIP(hipMemcpyAsync(perm.data(), u, vn * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    EMTD_HIP(hipStreamSynchronize(stream_));
+    for (int ip = 0; ip < npls_; ++ip)
+        for (int i = 0; i < n_; ++i) orig[size_t(ip) * n_ + i] = perm[size_t(ip) * n_ + S_->iperm[i]];
+    const std::string name = join_path(dir_, "v3." + std::to_string(layer));
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) throw Error(EMTD_ERR_NOFILE, "cannot write " + name);
+    const size_t w = std::fwrite(orig.data(), sizeof(double), vn, f);
+    std::fclose(f);
+    if (w != vn) throw Error(EMTD_ERR_NOFILE, "short write of " + name);
 }
 
 void CalcCore::begin() {
@@ -927,10 +947,19 @@ extern "C" int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info) {
 
 extern "C" void emtd_calc_destroy(emtd_calc* c) { delete c; }
 
+extern "C" int emtd_calc_set_snapshots(emtd_calc* c, int on) {
+    CALC_GUARD_BEGIN
+    if (!c) throw Error(EMTD_ERR_RANGE, "null handle");
+    c->core->set_snapshots(on != 0);
+    return EMTD_OK;
+    CALC_GUARD_END
+}
+
 extern "C" int emtd_calc_nonstat_3d_line(const char* workdir, int formulation, int scheme, int precision,
                                          int device) {
     emtd_calc* c = nullptr;
     int rc = emtd_calc_create(workdir, formulation, scheme, precision, device, &c);
+    if (rc == EMTD_OK) rc = emtd_calc_set_snapshots(c, 1);
     if (rc == EMTD_OK) rc = emtd_calc_run(c);
     if (rc == EMTD_OK) rc = emtd_calc_write_outputs(c);
     emtd_calc_destroy(c);

@@ -29,6 +29,7 @@ class CalcCore {
     int advance(int nsteps);  // runs up to nsteps steps (all when negative); synchronizes
     bool step();
     void set_timing(bool on);
+    void set_snapshots(bool on) { snapshots_ = on; }
     void collect() { collect_solve_times(); }
     void receiver_values(int kind, double* out) const;
     void last_field(double* out) const;
@@ -66,6 +67,7 @@ class CalcCore {
     const std::vector<double>& normal_layer(int tnum);
     void output_layer(int layer, const double* u);
     double* keep_layer(int layer, const double* u);
+    void write_snapshot(int layer, const double* u) const;
 
     std::string dir_;
     int device_ = -1, precision_ = 0, form_ = 0, scheme_ = 2;
@@ -114,6 +116,7 @@ class CalcCore {
     LoopState L_;
     DevBuf<double> bufA_, bufB_, bufC_, rhs_;
     bool timing_ = false;
+    bool snapshots_ = false;
     std::vector<hipEvent_t> ev_;
     size_t ev_used_ = 0;
     double solve_gpu_seconds_ = 0, factor_gpu_seconds_ = 0;

@@ -939,6 +939,15 @@ __global__ void __launch_bounds__(256) k_fwd_l21(const DevFront* __restrict__ F,
         if (r < m) {
             const T* lc = Lf + int64_t(c0) * m + r;
             int j = q;
+            // eight column loads in flight per thread (the upper levels have few row blocks,
+            // about one workgroup per CU, so each thread needs many loads to cover HBM latency)
+            for (; j + 28 < nc; j += 32) {
+                double a[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) a[k] = double(lc[int64_t(j + 4 * k) * m]);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc += a[k] * ys[j + 4 * k];
+            }
             for (; j + 12 < nc; j += 16) {
                 const double a0 = double(lc[int64_t(j) * m]), a1 = double(lc[int64_t(j + 4) * m]);
                 const double a2 = double(lc[int64_t(j + 8) * m]), a3 = double(lc[int64_t(j + 12) * m]);

@@ -220,7 +220,12 @@ ModelFiles read_model_files(const std::string& dir, int formulation) {
         if (pn) pn >> M.pname;
     }
     const int form = formulation >= 0 ? formulation : (M.variant_form >= 0 ? M.variant_form : 0);
-    if (form == 0)
+    // the initial edge DOFs themselves, when given, take the place of the nodal initial field
+    // (A2/time_approx.cpp:594-605 a0.edge, E2/time_approx.cpp:632-642 e0.edge)
+    const std::string edge0 = P(form == 0 ? "a0.edge" : "e0.edge");
+    if (file_exists(edge0))
+        M.u0_edges = read_bin<double>(edge0, size_t(M.n) * M.npls);
+    else if (form == 0)
         M.a0 = read_bin<double>(P("a0.dat"), size_t(M.npls) * M.kuzlov * 3);
     else
         M.v3a = read_bin<double>(P("v3a.dat"), size_t(M.npls) * M.kuzlov);

@@ -27,6 +27,7 @@ struct ModelFiles {
     double win_left = 0, win_right = 0;
     std::vector<double> a0;                  // npls*kuzlov*3 (A-formulation)
     std::vector<double> v3a;                 // npls*kuzlov (E-formulation)
+    std::vector<double> u0_edges;            // n*npls initial edge DOFs (a0.edge / e0.edge), else empty
     int variant_form = -1, variant_scheme = -1;  // from "emtd_variant" when present
     std::vector<char> anomal;                // n: edges carrying the normal field (mtrsizeforE0)
     int n_anom = 0;

@@ -116,6 +116,10 @@ int emtd_calc_step(emtd_calc* c, int nsteps, int* steps_done);
 /* Times every triangular solve with HIP events from now on (on != 0). */
 int emtd_calc_set_timing(emtd_calc* c, int on);
 int emtd_calc_write_outputs(emtd_calc* c);   /* Gather_edsall + tables into workdir */
+/* Writes v3.<t> (the edge field of layer t, n_edges x npls doubles, original numbering) for
+ * every layer kept for a later restart, as A2/time_approx.cpp:1093-1104 does (on != 0). Off by
+ * default; emtd_calc_nonstat_3d_line turns it on. */
+int emtd_calc_set_snapshots(emtd_calc* c, int on);
 /* Receiver tables after emtd_calc_run: kind 0 = B receivers (curl of U), 1 = E receivers (U);
  * out holds (t_last-t_first+1) x npls x nrec x 3 values, undifferentiated. */
 int emtd_calc_receiver_values(const emtd_calc* c, int kind, double* out);

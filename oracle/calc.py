@@ -84,7 +84,9 @@ def decade_plan(times: np.ndarray):
 
 # ----------------------------------------------------------------------------- initial state
 def initial_state(model: Model, form: str) -> np.ndarray:
-    """(npls, n) edge DOFs of the initial field."""
+    """(npls, n) edge DOFs of the initial field (``a0.edge`` / ``e0.edge`` when present)."""
+    if model.u0_edges and form in model.u0_edges:
+        return model.u0_edges[form].copy()
     v1 = model.edges[:, 0]
     v2 = model.edges[:, 1]
     axis = model.xyz[v2] - model.xyz[v1]

@@ -51,6 +51,7 @@ class Model:
     v3a: np.ndarray | None  # (npls, kuzlov)
     anomal: np.ndarray | None = None  # (n,) bool, ``mtrsizeforE0``
     pname: str | None = None          # ``PName``
+    u0_edges: dict | None = None      # {"A"|"E": (npls, n)} from ``a0.edge`` / ``e0.edge``
 
 
 def _read_flag(path, default, cast=float):
@@ -163,10 +164,16 @@ def read_model(d: str) -> Model:
         with open(j("PName")) as f:
             tok = f.read().split()
         pname = tok[0] if tok else None
+    # initial edge DOFs given directly (A2/time_approx.cpp:594-605, E2/time_approx.cpp:632-642):
+    # n*npls doubles, sources outermost
+    u0_edges = {}
+    for form, name in (("A", "a0.edge"), ("E", "e0.edge")):
+        if os.path.exists(j(name)):
+            u0_edges[form] = np.fromfile(j(name), dtype="<f8", count=n * npls).reshape(npls, n)
     return Model(kuz, kpar, xyz, nver, nvkat, sig, sig0, mu, l13d, edges, ed, times, recB, recE,
                  npls, rb, re_, fdirect, current, wires,
                  int(_read_flag(j("nSrsRazb"), 500, int)), int(_read_flag(j("n_neib"), 5, int)),
-                 window, a0, v3a, anomal, pname)
+                 window, a0, v3a, anomal, pname, u0_edges or None)
 
 
 def normal_field_name(pname: str, tnum: int) -> str:

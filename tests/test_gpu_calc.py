@@ -103,14 +103,51 @@ def test_calc_sweep_strategies_match_oracle(emtd, tmp_path, sweeps):
 
 
 def test_calc_drop_in_executable(emtd, tmp_path):
-    """The drop-in binary reads the directory and writes the three tables."""
+    """The drop-in binary reads the directory and writes the three tables, a.last and the
+    v3.<t> snapshots of the layers kept for restarts (A2/time_approx.cpp:1093-1111), which hold
+    the oracle's fields of those layers in original numbering."""
     import subprocess
-    d, _ = make_model(tmp_path, cfg=1, cells=(6, 6, 4), steps=(4, 2))
+    d, spec = make_model(tmp_path, cfg=1, cells=(6, 6, 4), steps=(4, 2))
     exe = os.path.join(os.path.dirname(emtd.LIB_PATH), "calcnonstat3dline")
     r = subprocess.run([exe, "-d", d], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     for stem in ("eall_anom.1", "edsall_anom.1", "ball_anom.1", "a.last", "DecIg", "nDec"):
         assert os.path.exists(os.path.join(d, stem)), stem
+    m = refio.read_model(d)
+    ref = calc.run(m, "A", 2, keep_fields=True)
+    kept = sorted({t for (tbeg, tend, _, _, nlst) in ref.plan for t in range(max(nlst, tbeg, 1), tend)})
+    assert kept
+    n = m.edges.shape[0]
+    for t in kept:
+        got = np.fromfile(os.path.join(d, f"v3.{t}"), dtype="<f8").reshape(m.npls, n)
+        want = ref.fields[t]
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("v3", t, "max rel err", err)
+        assert err < TOL64
+    last = np.fromfile(os.path.join(d, "a.last"), dtype="<f8").reshape(m.npls, n)
+    want = ref.fields[max(ref.fields)]
+    assert np.abs(last - want).max() / np.abs(want).max() < TOL64
+
+
+@pytest.mark.parametrize("form,scheme", [("A", 3), ("E", 2)])
+def test_calc_edge_initial_field_file(emtd, tmp_path, form, scheme):
+    """a0.edge / e0.edge (edge DOFs given directly, two sources) against the oracle."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(8, 8, 6), steps=(4, 2), form=form, scheme=scheme)
+    rng = np.random.default_rng(7)
+    m = refio.read_model(d)
+    n = m.edges.shape[0]
+    u0 = calc.initial_state(m, form) * (1.0 + 0.1 * rng.standard_normal((m.npls, n)))
+    u0.astype("<f8").tofile(os.path.join(d, "a0.edge" if form == "A" else "e0.edge"))
+    m = refio.read_model(d)
+    ref = calc.run(m, form, scheme)
+    c = emtd.Calc(d, 0 if form == "A" else 1, scheme, emtd.FP64)
+    c.run()
+    for kind, want in ((0, ref.valB), (1, ref.valE)):
+        got = c.receiver_values(kind)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("kind", kind, "receiver max rel err", err)
+        assert err < TOL64
+    c.close()
 
 
 TOL32 = 1e-5

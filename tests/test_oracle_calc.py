@@ -322,3 +322,23 @@ def test_primary_secondary_is_total_minus_normal(tmp_path, form, scheme, two):
         worst = max(worst, np.abs(w - ref).max() / scale)
     print("worst", worst)
     assert worst < 2e-5, worst
+
+
+@pytest.mark.parametrize("form", ["A", "E"])
+def test_edge_initial_field_file_is_the_projected_nodal_field(tmp_path, form):
+    """``a0.edge`` / ``e0.edge`` holding the edge projection of the nodal initial field (two
+    sources, sources outermost) must reproduce the run that starts from the nodal file."""
+    mg = modelgen()
+    d, spec = make_model(tmp_path / "nodal", cfg=1, cells=(6, 6, 4), steps=(3, 2), form=form)
+    spec.extra_wires = [((-10.0, -20.0, 0.0), (25.0, 15.0, 0.0))]
+    mg.write_model(spec, d)
+    m = refio.read_model(d)
+    r0 = calc.run(m, form, spec.scheme)
+    u0 = calc.initial_state(m, form)
+    assert u0.shape[0] == 2 and not np.allclose(u0[0], u0[1])
+    u0.astype("<f8").tofile(str(tmp_path / "nodal" / ("a0.edge" if form == "A" else "e0.edge")))
+    m2 = refio.read_model(d)
+    np.testing.assert_array_equal(calc.initial_state(m2, form), u0)
+    r1 = calc.run(m2, form, spec.scheme)
+    np.testing.assert_allclose(r1.valB, r0.valB, rtol=0, atol=1e-12 * np.abs(r0.valB).max())
+    np.testing.assert_allclose(r1.valE, r0.valE, rtol=0, atol=1e-12 * np.abs(r0.valE).max())
