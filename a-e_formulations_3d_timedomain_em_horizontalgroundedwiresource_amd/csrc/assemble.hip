@@ -32,29 +32,28 @@ __device__ __forceinline__ int ebit(int a, int ax) {
     return ax == (d + 1) % 3 ? (k & 1) : (k >> 1);
 }
 
-__device__ double local_c(int a, int b, const double h[3], double vol) {
+__device__ double local_c(int a, int b, const ElemData& d) {
     const int da = a >> 2, db = b >> 2;
     if (da != db) return 0.0;
     const int p = (da + 1) % 3, q = (da + 2) % 3;
-    return 4.0 * vol / (h[da] * h[da]) * m1(ebit(a, p), ebit(b, p)) * m1(ebit(a, q), ebit(b, q));
+    return d.vol4 * d.ih2[da] * m1(ebit(a, p), ebit(b, p)) * m1(ebit(a, q), ebit(b, q));
 }
 
-__device__ double local_b(int a, int b, const double h[3], double vol) {
+__device__ double local_b(int a, int b, const ElemData& d) {
     const int da = a >> 2, db = b >> 2;
     if (da == db) {
         const int p = (da + 1) % 3, q = (da + 2) % 3;
         const int ap = ebit(a, p), bp = ebit(b, p), aq = ebit(a, q), bq = ebit(b, q);
-        return 4.0 * vol / (h[da] * h[da]) *
-               (d1(ap, bp) * m1(aq, bq) / (h[p] * h[p]) + m1(ap, bp) * d1(aq, bq) / (h[q] * h[q]));
+        return d.vol4 * d.ih2[da] * (d1(ap, bp) * m1(aq, bq) * d.ih2[p] + m1(ap, bp) * d1(aq, bq) * d.ih2[q]);
     }
     if (db != (da + 1) % 3) {  // make b lie along a's first transverse axis
         const int t = a;
         a = b;
         b = t;
     }
-    const int d = a >> 2, p = (d + 1) % 3, q = (d + 2) % 3;
-    // b has transverse axes (q, d)
-    return -4.0 * vol / (h[d] * h[d] * h[p] * h[p]) * lp(ebit(a, p)) * m1(ebit(a, q), ebit(b, q)) * lp(ebit(b, d));
+    const int dd = a >> 2, p = (dd + 1) % 3, q = (dd + 2) % 3;
+    // b has transverse axes (q, dd)
+    return -d.vol4 * d.ih2[dd] * d.ih2[p] * lp(ebit(a, p)) * m1(ebit(a, q), ebit(b, q)) * lp(ebit(b, dd));
 }
 
 __global__ void k_element_table(int kpar, const double* __restrict__ xyz, const int* __restrict__ nver,
@@ -63,18 +62,25 @@ __global__ void k_element_table(int kpar, const double* __restrict__ xyz, const 
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kpar) return;
     const int n0 = nver[8 * e], n7 = nver[8 * e + 7];
-    ElemData d;
-    d.hx = xyz[3 * n7] - xyz[3 * n0];
-    d.hy = xyz[3 * n7 + 1] - xyz[3 * n0 + 1];
-    d.hz = xyz[3 * n7 + 2] - xyz[3 * n0 + 2];
+    const double hx = xyz[3 * n7] - xyz[3 * n0];
+    const double hy = xyz[3 * n7 + 1] - xyz[3 * n0 + 1];
+    const double hz = xyz[3 * n7 + 2] - xyz[3 * n0 + 2];
     const int mat = nvkat[e];
+    ElemData d;
+    d.ih2[0] = 1.0 / (hx * hx);
+    d.ih2[1] = 1.0 / (hy * hy);
+    d.ih2[2] = 1.0 / (hz * hz);
+    d.vol4 = 4.0 * hx * hy * hz;
     d.sigma = sigma[mat];
-    d.mu = mu[mat];
+    d.inv_mu = 1.0 / mu[mat];
     out[e] = d;
 }
 
-__global__ void k_assemble(int64_t nslots, const int64_t* __restrict__ cptr, const uint32_t* __restrict__ code,
-                           const ElemData* __restrict__ el, double* __restrict__ bval, double* __restrict__ cval) {
+// Slot s < nnz is lower entry s, slot nnz + i the diagonal entry i.
+__global__ void k_assemble(int64_t nnz, int64_t nslots, const int64_t* __restrict__ cptr,
+                           const uint32_t* __restrict__ code, const ElemData* __restrict__ el,
+                           double* __restrict__ b_ggl, double* __restrict__ b_di, double* __restrict__ c_ggl,
+                           double* __restrict__ c_di) {
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     double bs = 0.0, cs = 0.0;
@@ -83,13 +89,16 @@ __global__ void k_assemble(int64_t nslots, const int64_t* __restrict__ cptr, con
         const uint32_t e = c / 144u, ab = c % 144u;
         const int a = int(ab / 12u), b = int(ab % 12u);
         const ElemData d = el[e];
-        const double h[3] = {d.hx, d.hy, d.hz};
-        const double vol = d.hx * d.hy * d.hz;
-        bs += local_b(a, b, h, vol) / d.mu;
-        cs += local_c(a, b, h, vol) * d.sigma;
+        bs += local_b(a, b, d) * d.inv_mu;
+        cs += local_c(a, b, d) * d.sigma;
     }
-    bval[s] = bs;
-    cval[s] = cs;
+    if (s < nnz) {
+        b_ggl[s] = bs;
+        c_ggl[s] = cs;
+    } else {
+        b_di[s - nnz] = bs;
+        c_di[s - nnz] = cs;
+    }
 }
 
 }  // namespace
@@ -164,40 +173,48 @@ Assembler::Assembler(const emtd_mesh& m, int n, const int* ig, const int* jg, hi
     nver.upload(m.nver, size_t(m.kpar) * 8, s);
     nvkat.upload(m.nvkat, m.kpar, s);
     el_.alloc(m.kpar);
+    kpar_ = m.kpar;
     hipLaunchKernelGGL(k_element_table, dim3((m.kpar + 255) / 256), dim3(256), 0, s, m.kpar, xyz.p, nver.p, nvkat.p,
                        sig.p, mu.p, el_.p);
     EMTD_HIP(hipGetLastError());
     EMTD_HIP(hipStreamSynchronize(s));
 }
 
+Assembler::~Assembler() {
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
 void Assembler::run(double* d_b_ggl, double* d_b_di, double* d_c_ggl, double* d_c_di) {
-    // the slot space is [lower | diagonal]; kernel writes into one contiguous scratch then splits
     const int64_t ns = nnz_ + n_;
-    if (bscratch_.n < size_t(ns)) { bscratch_.alloc(ns); cscratch_.alloc(ns); }
-    hipLaunchKernelGGL(k_assemble, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, stream, ns, cptr_.p, codes_.p,
-                       el_.p, bscratch_.p, cscratch_.p);
+    if (!ev_[0]) {
+        EMTD_HIP(hipEventCreate(&ev_[0]));
+        EMTD_HIP(hipEventCreate(&ev_[1]));
+    }
+    EMTD_HIP(hipEventRecord(ev_[0], stream));
+    hipLaunchKernelGGL(k_assemble, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, stream, nnz_, ns, cptr_.p,
+                       codes_.p, el_.p, d_b_ggl, d_b_di, d_c_ggl, d_c_di);
     EMTD_HIP(hipGetLastError());
-    EMTD_HIP(hipMemcpyAsync(d_b_ggl, bscratch_.p, nnz_ * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    EMTD_HIP(hipMemcpyAsync(d_c_ggl, cscratch_.p, nnz_ * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    EMTD_HIP(hipMemcpyAsync(d_b_di, bscratch_.p + nnz_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    EMTD_HIP(hipMemcpyAsync(d_c_di, cscratch_.p + nnz_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    EMTD_HIP(hipEventRecord(ev_[1], stream));
+    timed_ = true;
+}
+
+double Assembler::last_seconds() {
+    if (!timed_) return 0.0;
+    EMTD_HIP(hipEventSynchronize(ev_[1]));
+    float ms = 0;
+    EMTD_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    return ms * 1e-3;
+}
+
+int64_t Assembler::bytes() const {
+    const int64_t ns = nnz_ + n_;
+    return ns * 16 + (ns + 1) * 8 + int64_t(codes_.n) * 4 + int64_t(kpar_) * int64_t(sizeof(ElemData));
 }
 
 double Assembler::run_timed(double* b_all, double* c_all) {
-    hipEvent_t e0, e1;
-    EMTD_HIP(hipEventCreate(&e0));
-    EMTD_HIP(hipEventCreate(&e1));
-    const int64_t ns = nnz_ + n_;
-    EMTD_HIP(hipEventRecord(e0, stream));
-    hipLaunchKernelGGL(k_assemble, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, stream, ns, cptr_.p, codes_.p,
-                       el_.p, b_all, c_all);
-    EMTD_HIP(hipEventRecord(e1, stream));
-    EMTD_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    EMTD_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return ms * 1e-3;
+    run(b_all, b_all + nnz_, c_all, c_all + nnz_);
+    return last_seconds();
 }
 
 }  // namespace emtd

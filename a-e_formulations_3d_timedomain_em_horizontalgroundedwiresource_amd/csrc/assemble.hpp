@@ -12,8 +12,12 @@
 
 namespace emtd {
 
+// Per-element factors of the closed-form local matrices (no division in the assembly loop).
 struct ElemData {
-    double hx, hy, hz, sigma, mu;
+    double ih2[3];   // 1 / h_d^2
+    double vol4;     // 4 * hx * hy * hz
+    double sigma;
+    double inv_mu;
 };
 
 // Contribution lists per slot (lower entries first, then the diagonal), in element order.
@@ -23,10 +27,15 @@ void build_contributions(int kpar, const int* ed, int n, const int* ig, const in
 class Assembler {
   public:
     Assembler(const emtd_mesh& m, int n, const int* ig, const int* jg, hipStream_t s);
+    ~Assembler();
     // Writes B (curl-curl/mu) and C (sigma mass) into device arrays (lower RSF + diagonal).
     void run(double* d_b_ggl, double* d_b_di, double* d_c_ggl, double* d_c_di);
     // Runs only the assembly kernel into [lower | diagonal] arrays; returns its GPU time.
     double run_timed(double* b_all, double* c_all);
+    // GPU time of the last run() (waits for it) and the kernel's algorithmic bytes: 16 B
+    // written per slot, the 8 B slot pointer, 4 B per element contribution, the element table.
+    double last_seconds();
+    int64_t bytes() const;
     int64_t nnz() const { return nnz_; }
     int64_t contributions() const { return int64_t(codes_.n); }
     hipStream_t stream;
@@ -37,7 +46,9 @@ class Assembler {
     DevBuf<int64_t> cptr_;
     DevBuf<uint32_t> codes_;
     DevBuf<ElemData> el_;
-    DevBuf<double> bscratch_, cscratch_;
+    int kpar_ = 0;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    bool timed_ = false;
 };
 
 }  // namespace emtd

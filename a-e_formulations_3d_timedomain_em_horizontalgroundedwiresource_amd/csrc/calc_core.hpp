@@ -132,6 +132,7 @@ class CalcCore {
     int64_t k_nnz_ = 0;
     int64_t footprint_ = 0;
     double setup_seconds_ = 0, factor_seconds_ = 0, step_seconds_ = 0;
+    double assembly_gpu_seconds_ = 0, assembly_bytes_ = 0;
     int64_t steps_ = 0, n_factorizations_ = 0, refine_iterations_ = 0, fallbacks_ = 0;
 };
 

@@ -238,6 +238,13 @@ def timed_leg(emtd, d, spec, precision, args):
         out["roofline"], out["roofline_other"] = r_fact, r_solve
     else:
         out["roofline"], out["roofline_other"] = r_solve, r_fact
+    # the assembly of B and C (once per model, outside the timed region)
+    if ci.assembly_gpu_seconds > 0:
+        ga = ci.assembly_bytes / ci.assembly_gpu_seconds / 1e9
+        out["roofline_assembly"] = {"bound": "hbm", "achieved": round(ga, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": round(ga / HBM_PEAK_GBS, 4), "bytes_per_launch": ci.assembly_bytes,
+                                    "launch_ms": round(ci.assembly_gpu_seconds * 1e3, 4),
+                                    "kernel": "k_assemble: B and C in lower RSF, one thread per slot"}
     c.close()
     return {"out": out, "receivers": receivers}
 

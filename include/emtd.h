@@ -146,6 +146,9 @@ typedef struct emtd_calc_info {
     int64_t nnz_low;             /* strictly lower entries of the RSF pattern */
     int done;                    /* 1 when every time layer has been computed */
     double last_residual;        /* FP32_IR: last relative residual ||b-Kx||/||b|| */
+    double assembly_gpu_seconds; /* GPU time of the B and C assembly kernel (HIP events) */
+    double assembly_bytes;       /* its algorithmic bytes: 16 per RSF slot written, 8 per slot
+                                    pointer, 4 per element contribution, the element table */
 } emtd_calc_info;
 int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
 void emtd_calc_destroy(emtd_calc* c);
