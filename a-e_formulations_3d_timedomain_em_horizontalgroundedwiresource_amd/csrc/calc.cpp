@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <map>
 #include <memory>
@@ -25,6 +26,7 @@
 #include "model_io.hpp"
 #include "step.hpp"
 #include "symbolic.hpp"
+#include "tune.hpp"
 #include "vec.hpp"
 
 using namespace emtd;
@@ -40,6 +42,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     form_ = form >= 0 ? form : (M_.variant_form >= 0 ? M_.variant_form : 0);
     scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
     if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
+    if (const char* e = std::getenv("EMTD_AUTOTUNE")) autotune_ = std::atoi(e) != 0;
     n_ = M_.n;
     npls_ = M_.npls;
     // host: pattern, ordering and symbolic analysis (Dirichlet edges are singletons)
@@ -356,6 +359,37 @@ void CalcCore::factorize(double mult) {
     if (bad >= 0) throw Error(EMTD_ERR_NOTSPD, "non-positive pivot at permuted index " + std::to_string(bad));
     factor_seconds_ += wall_seconds() - t0;
     ++n_factorizations_;
+    if (autotune_ && !tuned_) run_autotune(mult);
+    tuning_used_ = use64_ ? c64_->tuning() : c32_->tuning();
+}
+
+// Picks the launch parameters of the solver in use on this model's first matrix (or takes them
+// from the process-wide cache for a model of the same structure). The tuning refactorizations
+// and solves are not counted in the factorization and solve statistics.
+void CalcCore::run_autotune(double mult) {
+    const double t0 = wall_seconds();
+    tuned_ = true;
+    auto go = [&](auto& C, int elem) {
+        const std::string key = tuning_key(*S_, elem), cls = tuning_class_key(*S_, elem);
+        Tuning t;
+        if (tuning_cache_get(key, &t) || tuning_cache_get(cls, &t)) {
+            C.set_tuning(t);
+            tune_report_ = "from the cache\n";
+            return;
+        }
+        DevBuf<double> x(size_t(n_) * npls_);
+        auto refactor = [&]() {
+            C.factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
+            return C.check_info() < 0;
+        };
+        t = autotune(C, refactor, u0_.p, x.p, npls_, n_, &tune_report_);
+        tuning_cache_put(key, t);
+        tuning_cache_put(cls, t);
+    };
+    if (use64_) go(*c64_, 8);
+    else go(*c32_, 4);
+    tune_seconds_ = wall_seconds() - t0;
+    if (std::getenv("EMTD_AUTOTUNE_VERBOSE")) std::fprintf(stderr, "[emtd autotune] %s", tune_report_.c_str());
 }
 
 void CalcCore::raw_solve(const double* b, double* x) {
@@ -846,6 +880,12 @@ void CalcCore::info(emtd_calc_info* inf) const {
     inf->last_residual = last_residual_;
     inf->assembly_gpu_seconds = assembly_gpu_seconds_;
     inf->assembly_bytes = assembly_bytes_;
+    inf->tuning[0] = tuning_used_.outer_block;
+    inf->tuning[1] = tuning_used_.wide_tile_cols;
+    inf->tuning[2] = tuning_used_.lookahead_lds_pad;
+    inf->tuning[3] = tuning_used_.split_level_fronts;
+    inf->tuning[4] = tuning_used_.small_front_sweeps;
+    inf->tune_seconds = tune_seconds_;
     inf->factorizations = n_factorizations_;
     inf->factor_gpu_seconds = factor_gpu_seconds_;
     inf->solves = n_solves_;

@@ -58,6 +58,7 @@ class CalcCore {
     void collect_solve_times();
     bool stop_requested() const;
     void factorize(double mult);
+    void run_autotune(double mult);
     void solve(const double* b, double* x, const double* predictor);
     void raw_solve(const double* b, double* x);
     void refine(const double* b, double* x, const double* predictor);
@@ -117,6 +118,10 @@ class CalcCore {
     DevBuf<double> bufA_, bufB_, bufC_, rhs_;
     bool timing_ = false;
     bool snapshots_ = false;
+    bool autotune_ = false, tuned_ = false;  // EMTD_AUTOTUNE: tune at the first factorization
+    Tuning tuning_used_;
+    double tune_seconds_ = 0;
+    std::string tune_report_;
     std::vector<hipEvent_t> ev_;
     size_t ev_used_ = 0;
     double solve_gpu_seconds_ = 0, factor_gpu_seconds_ = 0;

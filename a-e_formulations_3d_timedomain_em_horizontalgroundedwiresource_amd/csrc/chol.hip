@@ -33,7 +33,6 @@ namespace emtd {
 namespace {
 
 constexpr int NB = 32;    // panel width of the blocked factorization
-constexpr int NBO = 128;  // outer block of the factorization
 constexpr int SB = 64;    // column block of the triangular sweeps
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -1107,8 +1106,31 @@ inline unsigned grid_for(int64_t n, int block) {
 
 }  // namespace
 
+Tuning default_tuning() {
+    Tuning t;
+    if (const char* e = std::getenv("EMTD_OUTER_BLOCK")) t.outer_block = std::max(1, std::atoi(e) / NB) * NB;
+    if (const char* e = std::getenv("EMTD_WIDE_TILE_COLS")) t.wide_tile_cols = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("EMTD_LOOKAHEAD_LDS_PAD")) t.lookahead_lds_pad = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("EMTD_SPLIT_LEVEL_FRONTS")) t.split_level_fronts = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("EMTD_SMALL_FRONT_SWEEPS")) t.small_front_sweeps = std::atoi(e) != 0;
+    return t;
+}
+
+template <typename T>
+void GpuCholesky<T>::set_tuning(const Tuning& t) {
+    Tuning n = t;
+    n.outer_block = std::max(NB, (n.outer_block / NB) * NB);
+    n.lookahead_lds_pad = std::min(std::max(0, n.lookahead_lds_pad), 60 * 1024);
+    if (n.split_level_fronts != tune_.split_level_fronts || n.small_front_sweeps != tune_.small_front_sweeps) {
+        big_ready_ = false;
+        inv_stale_ = true;
+    }
+    tune_ = n;
+}
+
 template <typename T>
 GpuCholesky<T>::GpuCholesky(const Symbolic& S, hipStream_t stream) : S_(S), stream_(stream) {
+    tune_ = default_tuning();
     std::vector<DevFront> fr(S.nf);
     for (int f = 0; f < S.nf; ++f) {
         DevFront& d = fr[f];
@@ -1192,7 +1214,7 @@ void GpuCholesky<T>::update(hipStream_t st, const int* fz, int cnt, int k0, int 
                             int cols, size_t lds_pad) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
-    if (cols >= 256) {
+    if (cols >= tune_.wide_tile_cols) {
         const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
         hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), lds_pad, st, fronts_.p, fz, k0,
                            k1, cmin, cmax, L_.p, U_.p);
@@ -1216,7 +1238,7 @@ void GpuCholesky<T>::extend_add(int l, const std::vector<int>& ptr, const int2* 
 
 // Look-ahead: the trailing update after outer block kb is split into the next outer block's
 // columns (on the main stream, right away) and the rest (on a second stream), so the panel
-// chain of block kb+NBO runs while the bulk of the update is still in flight. The next trailing
+// chain of the next outer block runs while the bulk of the update is still in flight. The next trailing
 // update writes the same columns as the bulk, so it waits for it (event), and the main stream
 // waits for the second one before the update block and at the end of every level.
 template <typename T>
@@ -1227,13 +1249,11 @@ void GpuCholesky<T>::numeric() {
         EMTD_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
         EMTD_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     }
-    // The bulk update on the second stream asks for extra LDS so that at most one of its
-    // 1024-thread workgroups runs per CU: the other half of each CU's registers stays free for
-    // the panel chain of the main stream, which otherwise waits for the bulk update to drain.
-    if (const char* e = std::getenv("EMTD_LOOKAHEAD_LDS_PAD")) side_lds_pad_ = size_t(std::max(0, std::atoi(e)));
-    // outer block (a multiple of the panel width; tuning override EMTD_OUTER_BLOCK)
-    int nbo = NBO;
-    if (const char* e = std::getenv("EMTD_OUTER_BLOCK")) nbo = std::max(1, std::atoi(e) / NB) * NB;
+    // The bulk update on the second stream may ask for extra LDS so that at most one of its
+    // 1024-thread workgroups runs per CU, leaving the other half of each CU's registers to the
+    // panel chain of the main stream.
+    const size_t side_lds_pad = size_t(tune_.lookahead_lds_pad);
+    const int nbo = tune_.outer_block;
     bool side_pending = false;
     auto join_side = [&]() {
         if (!side_pending) return;
@@ -1273,7 +1293,7 @@ void GpuCholesky<T>::numeric() {
                 if (mp > k2) {
                     EMTD_HIP(hipEventRecord(ev_main_, stream_));
                     EMTD_HIP(hipStreamWaitEvent(side_, ev_main_, 0));
-                    update(side_, fz, cnt, kb, k1, k2, INT_MAX, mm - k2, mp - k2, side_lds_pad_);
+                    update(side_, fz, cnt, kb, k1, k2, INT_MAX, mm - k2, mp - k2, side_lds_pad);
                     side_pending = true;
                 }
                 // next outer block's columns [kb+nbo, kb+2nbo), K = nbo, all rows
@@ -1357,11 +1377,8 @@ void GpuCholesky<T>::prepare_big_levels() {
     const Symbolic& S = S_;
     big_.assign(S.nlevels, 0);
     lv_mcap_.assign(S.nlevels, 0);
-    // sweep strategy overrides (tuning and tests): levels with fewer fronts than
-    // EMTD_SPLIT_LEVEL_FRONTS use the split sweeps (0: none); EMTD_SMALL_FRONT_SWEEPS=0 turns off
-    // the one-wavefront-per-front sweeps
-    if (const char* e = std::getenv("EMTD_SPLIT_LEVEL_FRONTS")) big_level_fronts_ = std::max(0, std::atoi(e));
-    if (const char* e = std::getenv("EMTD_SMALL_FRONT_SWEEPS")) small_sweeps_ = std::atoi(e) != 0;
+    // sweep strategy: levels with fewer fronts than split_level_fronts use the split sweeps (0:
+    // none); small_front_sweeps = 0 turns off the one-wavefront-per-front sweeps
     std::vector<int2> fc, l21, bc;
     std::vector<int3> l21t;
     std::vector<int> foff(S.nf, 0);
@@ -1377,7 +1394,7 @@ void GpuCholesky<T>::prepare_big_levels() {
     lv_rows_.assign(S.nlevels + 1, 0);
     for (int l = 0; l < S.nlevels; ++l) {
         const int nfl = S.level_ptr[l + 1] - S.level_ptr[l];
-        big_[l] = nfl < big_level_fronts_ ? 1 : 0;
+        big_[l] = nfl < tune_.split_level_fronts ? 1 : 0;
         // one wavefront per front when every front has at most kSmallP pivots and four fronts'
         // rows fit in 48 KB of LDS
         int maxp = 0, maxm = 0;
@@ -1385,7 +1402,7 @@ void GpuCholesky<T>::prepare_big_levels() {
             maxp = std::max(maxp, S.pcount[S.level_fronts[k]]);
             maxm = std::max(maxm, S.mcount[S.level_fronts[k]]);
         }
-        lv_mcap_[l] = (small_sweeps_ && !big_[l] && maxp <= kSmallP && maxm <= 1536) ? std::max(maxm, kSmallP) : 0;
+        lv_mcap_[l] = (tune_.small_front_sweeps && !big_[l] && maxp <= kSmallP && maxm <= 1536) ? std::max(maxm, kSmallP) : 0;
         if (big_[l]) {
             int off = 0, maxnp = 0;
             int64_t po = 0;
@@ -1500,7 +1517,10 @@ void GpuCholesky<T>::prepare_inverses() {
 
 template <typename T>
 void GpuCholesky<T>::invert_diagonal_blocks() {
-    if (!inv_.p) prepare_inverses();
+    if (!inv_.p || inv_stale_) {
+        prepare_inverses();
+        inv_stale_ = false;
+    }
     if (n_inv_items_ > 0)
         hipLaunchKernelGGL(k_invdiag<T>, dim3(n_inv_items_), dim3(64), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
                            L_.p, inv_.p);
@@ -1528,6 +1548,7 @@ template <typename T>
 void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld) {
     const Symbolic& S = S_;
     if (!big_ready_) prepare_big_levels();
+    if (inv_stale_) invert_diagonal_blocks();  // the sweep plan changed after the factorization
     if (nrhs > wrhs_) {
         W_.alloc(size_t(std::max<int64_t>(S.wsize, 1)) * nrhs);
         wrhs_ = nrhs;

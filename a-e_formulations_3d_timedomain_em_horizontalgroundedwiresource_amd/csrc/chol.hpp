@@ -33,6 +33,25 @@ struct DevFront {
     int child_beg, child_end;
 };
 
+// Launch parameters of the factorization and of the triangular sweeps. Every choice gives the
+// same factor up to rounding; the best one depends on the front sizes (see tune.hpp).
+struct Tuning {
+    int outer_block = 128;          // columns per outer block (a multiple of the 32-column panel)
+    int wide_tile_cols = 256;       // update targets at least this wide use 128x128 tiles
+    int lookahead_lds_pad = 0;      // extra LDS bytes of the look-ahead bulk update
+    int split_level_fronts = 1024;  // levels with fewer fronts use the split sweeps
+    int small_front_sweeps = 1;     // one wavefront per front where every front has <= 64 pivots
+    bool operator==(const Tuning& o) const {
+        return outer_block == o.outer_block && wide_tile_cols == o.wide_tile_cols &&
+               lookahead_lds_pad == o.lookahead_lds_pad && split_level_fronts == o.split_level_fronts &&
+               small_front_sweeps == o.small_front_sweeps;
+    }
+};
+
+// The defaults, with the environment overrides EMTD_OUTER_BLOCK, EMTD_WIDE_TILE_COLS,
+// EMTD_LOOKAHEAD_LDS_PAD, EMTD_SPLIT_LEVEL_FRONTS and EMTD_SMALL_FRONT_SWEEPS applied.
+Tuning default_tuning();
+
 // Device copy of the symbolic structure plus factor storage of type T (double or float).
 template <typename T>
 class GpuCholesky {
@@ -58,6 +77,10 @@ class GpuCholesky {
     // Empty, or the description of the split-sweep wait that hit its limit (synchronizes).
     std::string sweep_error();
     double last_factor_seconds() const { return factor_seconds_; }
+    // Launch parameters. A change of the sweep parameters rebuilds the sweep plan and the
+    // inverted pivot blocks (from the current factor) before the next solve.
+    const Tuning& tuning() const { return tune_; }
+    void set_tuning(const Tuning& t);
 
     const Symbolic& sym() const { return S_; }
     hipStream_t stream() const { return stream_; }
@@ -85,10 +108,9 @@ class GpuCholesky {
     std::vector<int> level_sorted_off_;
     DevBuf<int> info_, panel_done_;
     int wrhs_ = 0;
-    int big_level_fronts_ = 1024;  // levels with fewer fronts use the split sweeps
-    size_t side_lds_pad_ = 0;       // extra LDS of the look-ahead bulk update (one workgroup per CU)
-    bool small_sweeps_ = true;     // one wavefront per front where every front has <= 64 pivots
+    Tuning tune_;
     bool big_ready_ = false;
+    bool inv_stale_ = false;  // sweep plan changed: inverted pivot blocks must be recomputed
     std::vector<char> big_;
     std::vector<int> lv_mcap_;  // > 0: the level uses the one-wavefront-per-front sweeps (max rows)
     std::vector<int> lv_fc_, lv_l21_, lv_l21t_, lv_bc_;

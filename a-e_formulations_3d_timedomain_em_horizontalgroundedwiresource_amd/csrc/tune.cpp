@@ -1,0 +1,157 @@
+// Autotuner of the solver's launch parameters (see tune.hpp).
+#include "tune.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <map>
+#include <mutex>
+
+#include "../../include/emtd.h"
+#include "common.hpp"
+
+namespace emtd {
+
+namespace {
+
+std::mutex g_cache_mutex;
+std::map<std::string, Tuning>& cache() {
+    static std::map<std::string, Tuning> c;
+    return c;
+}
+
+std::string describe(const Tuning& t) {
+    char b[160];
+    std::snprintf(b, sizeof b, "outer_block=%d wide_tile_cols=%d lookahead_lds_pad=%d split_level_fronts=%d "
+                  "small_front_sweeps=%d", t.outer_block, t.wide_tile_cols, t.lookahead_lds_pad,
+                  t.split_level_fronts, t.small_front_sweeps);
+    return b;
+}
+
+}  // namespace
+
+std::string tuning_key(const Symbolic& S, int elem_bytes) {
+    char b[200];
+    std::snprintf(b, sizeof b, "%d:%d:%d:%d:%d:%.17g:%.17g", elem_bytes, S.n, S.nn, S.nf, S.nlevels, S.nnz_l, S.flops);
+    return b;
+}
+
+std::string tuning_class_key(const Symbolic& S, int elem_bytes) {
+    char b[64];
+    const int bin = int(std::floor(2.0 * std::log2(double(std::max(S.nn, 1)))));
+    std::snprintf(b, sizeof b, "class:%d:%d:%d", elem_bytes, S.nlevels, bin);
+    return b;
+}
+
+bool tuning_cache_get(const std::string& key, Tuning* t) {
+    std::lock_guard<std::mutex> g(g_cache_mutex);
+    auto it = cache().find(key);
+    if (it == cache().end()) return false;
+    *t = it->second;
+    return true;
+}
+
+void tuning_cache_put(const std::string& key, const Tuning& t) {
+    std::lock_guard<std::mutex> g(g_cache_mutex);
+    cache()[key] = t;
+}
+
+template <typename T>
+Tuning autotune(GpuCholesky<T>& C, const std::function<bool()>& refactor, const double* d_b, double* d_x, int nrhs,
+                int64_t ld, std::string* report) {
+    constexpr double kInf = std::numeric_limits<double>::infinity();
+    std::string log;
+    auto note = [&](const char* what, const Tuning& t, double s) {
+        char b[64];
+        std::snprintf(b, sizeof b, "%s %.3f ms: ", what, s * 1e3);
+        log += b + describe(t) + "\n";
+    };
+    // factorization parameters
+    auto time_factor = [&](const Tuning& t) {
+        C.set_tuning(t);
+        const double s = refactor() ? C.last_factor_seconds() : kInf;
+        note("factor", t, s);
+        return s;
+    };
+    Tuning best = C.tuning();
+    double bf = time_factor(best);
+    bf = std::min(bf, time_factor(best));
+    auto try_factor = [&](const Tuning& t) {
+        if (t == best) return;
+        const double s = time_factor(t);
+        if (s < 0.98 * bf) { bf = s; best = t; }
+    };
+    const Tuning base = best;
+    for (int ob : {64, 128, 256}) {
+        Tuning t = best;
+        t.outer_block = ob;
+        try_factor(t);
+    }
+    for (int wt : {128, 256, 512}) {
+        Tuning t = best;
+        t.wide_tile_cols = wt;
+        try_factor(t);
+    }
+    {
+        Tuning t = best;
+        t.lookahead_lds_pad = best.lookahead_lds_pad ? 0 : 48 * 1024;
+        try_factor(t);
+    }
+    C.set_tuning(best);
+    if (!refactor()) throw Error(EMTD_ERR_NOTSPD, "autotune: the chosen factorization met a non-positive pivot");
+    // sweep strategy, timed on the factor of the chosen parameters
+    hipEvent_t e[4];
+    for (hipEvent_t& x : e) EMTD_HIP(hipEventCreate(&x));
+    auto time_solve = [&](const Tuning& t) {
+        C.set_tuning(t);
+        C.solve(d_b, d_x, nrhs, ld);  // builds the plan and the inverted pivot blocks
+        double s = kInf;
+        for (int k = 0; k < 3; ++k) {
+            EMTD_HIP(hipEventRecord(e[0], C.stream()));
+            C.solve(d_b, d_x, nrhs, ld);
+            EMTD_HIP(hipEventRecord(e[1], C.stream()));
+            EMTD_HIP(hipEventSynchronize(e[1]));
+            float ms = 0;
+            EMTD_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+            s = std::min(s, double(ms) * 1e-3);
+        }
+        const std::string err = C.sweep_error();
+        if (!err.empty()) throw Error(EMTD_ERR_ERROR, "autotune: " + err);
+        note("solve", t, s);
+        return s;
+    };
+    double bs = time_solve(best);
+    auto try_solve = [&](const Tuning& t) {
+        if (t == best) return;
+        const double s = time_solve(t);
+        if (s < 0.98 * bs) { bs = s; best = t; }
+    };
+    for (int sl : {0, 256, 1024, 4096}) {
+        Tuning t = best;
+        t.split_level_fronts = sl;
+        try_solve(t);
+    }
+    {
+        Tuning t = best;
+        t.small_front_sweeps = !best.small_front_sweeps;
+        try_solve(t);
+    }
+    for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    C.set_tuning(best);
+    C.solve(d_b, d_x, nrhs, ld);  // the chosen plan is in place before the caller's first solve
+    EMTD_HIP(hipStreamSynchronize(C.stream()));
+    log += "chosen (factor " + std::to_string(bf * 1e3) + " ms, solve " + std::to_string(bs * 1e3) +
+           " ms; start " + describe(base) + "): " + describe(best) + "\n";
+    if (report) *report = log;
+    return best;
+}
+
+template Tuning autotune<double>(GpuCholesky<double>&, const std::function<bool()>&, const double*, double*, int,
+                                 int64_t, std::string*);
+template Tuning autotune<float>(GpuCholesky<float>&, const std::function<bool()>&, const double*, double*, int,
+                                int64_t, std::string*);
+
+}  // namespace emtd

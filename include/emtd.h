@@ -149,6 +149,10 @@ typedef struct emtd_calc_info {
     double assembly_gpu_seconds; /* GPU time of the B and C assembly kernel (HIP events) */
     double assembly_bytes;       /* its algorithmic bytes: 16 per RSF slot written, 8 per slot
                                     pointer, 4 per element contribution, the element table */
+    int tuning[5];               /* launch parameters in use: outer block, 128x128-tile threshold
+                                    (target columns), look-ahead LDS pad, split-sweep threshold
+                                    (fronts per level), one-wavefront-per-front sweeps (0/1) */
+    double tune_seconds;         /* time spent by the autotuner (environment EMTD_AUTOTUNE=1) */
 } emtd_calc_info;
 int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
 void emtd_calc_destroy(emtd_calc* c);

@@ -273,3 +273,33 @@ def test_calc_primary_secondary_missing_layer_fails(emtd, mg, tmp_path):
         c.close()
     finally:
         mg.remove_normal_fields(spec.primary_secondary, len(spec.times))
+
+
+def test_calc_autotuned_matches_oracle(emtd, tmp_path, monkeypatch):
+    """EMTD_AUTOTUNE=1: the launch parameters are chosen by timing candidates on the model's
+    first matrix; the results stay at the FP64 bar, and a second model of the same structure
+    takes the parameters from the cache."""
+    monkeypatch.setenv("EMTD_AUTOTUNE", "1")
+    d, spec = make_model(tmp_path, cfg=3, cells=(12, 11, 9), steps=(4, 2), form="E", scheme=3)
+    m = refio.read_model(d)
+    ref = calc.run(m, "E", 3)
+    c = emtd.Calc(d, 1, 3, emtd.FP64)
+    c.run()
+    inf = c.info()
+    tuning = list(inf.tuning)
+    print("tuning", tuning, "tune seconds", inf.tune_seconds, flush=True)
+    assert inf.tune_seconds > 0
+    assert tuning[0] in (64, 128, 256) and tuning[1] in (128, 256, 512)
+    assert tuning[3] in (0, 256, 1024, 4096) and tuning[4] in (0, 1)
+    for kind, want in ((0, ref.valB), (1, ref.valE)):
+        got = c.receiver_values(kind)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("kind", kind, "receiver max rel err", err)
+        assert err < TOL64
+    c.close()
+    c2 = emtd.Calc(d, 1, 3, emtd.FP64)
+    c2.step(1)
+    inf2 = c2.info()
+    assert list(inf2.tuning) == tuning
+    assert inf2.tune_seconds < inf.tune_seconds
+    c2.close()
