@@ -43,6 +43,8 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
     if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
     if (const char* e = std::getenv("EMTD_AUTOTUNE")) autotune_ = std::atoi(e) != 0;
+    prefetch_ = precision_ == EMTD_FP64;
+    if (const char* e = std::getenv("EMTD_PREFETCH_FACTOR")) prefetch_ = prefetch_ && std::atoi(e) != 0;
     n_ = M_.n;
     npls_ = M_.npls;
     // host: pattern, ordering and symbolic analysis (Dirichlet edges are singletons)
@@ -82,9 +84,14 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         dest_off_ = to_device(doff, stream_);
         dest_diag_ = to_device(ddiag, stream_);
     }
-    if (precision_ == EMTD_FP64)
+    if (precision_ == EMTD_FP64) {
         c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
-    else
+        if (prefetch_) {
+            EMTD_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
+            EMTD_HIP(hipEventCreateWithFlags(&swap_ev_, hipEventDisableTiming));
+            c64b_ = std::make_unique<GpuCholesky<double>>(*S_, fstream_);
+        }
+    } else
         c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
     perm_ = to_device(S_->perm, stream_);
     if (precision_ == EMTD_FP32_IR) build_residual_operator();
@@ -247,8 +254,12 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
 CalcCore::~CalcCore() {
     try {
         select_device(device_);
+        if (fstream_) (void)hipStreamSynchronize(fstream_);
+        c64b_.reset();
         c64_.reset();
         c32_.reset();
+        if (swap_ev_) (void)hipEventDestroy(swap_ev_);
+        if (fstream_) (void)hipStreamDestroy(fstream_);
         stored_.clear();
         for (int k = 0; k < 2; ++k) {
             if (en_ev_[k]) {
@@ -277,7 +288,7 @@ int64_t CalcCore::estimate_footprint() const {
     if (M_.n_anom > 0 && !M_.fdirect)
         b += (2 * nnz + n) * 8 + n * np * 8 + int64_t(M_.n_anom) * (np * 8 + 4);  // C_dsig, E_n
     if (precision_ == EMTD_FP64) {
-        b += factor(8);
+        b += factor(8) * (prefetch_ ? 2 : 1);  // the second factor of the look-ahead factorization
     } else {
         b += factor(4) + factor(8);                   // FP32 factor and the FP64 fallback
         b += (2 * nnz + n) * 20 + nn * 16 + 2 * n * np * 8;  // residual operator, IR vectors
@@ -328,11 +339,85 @@ void CalcCore::build_residual_operator() {
     if (const char* e = std::getenv("EMTD_IR_TOL")) ir_tol_ = std::atof(e);
 }
 
+// The mult of every factorization of the run, in order, as the time loop will ask for them:
+// per decade, 1/dt (2-point) or (dt+dt0)/(dt dt0) (3-point) from the decade's first step and
+// the layers it restarts from; the 3-point scheme's first decade also factors 1/(t1 - t0) for its
+// two backward-Euler start steps (A2/time_approx.cpp:967-1006, A3/time_approx.cpp:864-945).
+std::vector<double> CalcCore::factorization_mults() const {
+    const std::vector<double>& t = M_.times;
+    std::vector<double> out;
+    for (const Decade& D : plan_) {
+        if (1 >= D.tbeg && 1 < D.tend) {
+            out.push_back(1.0 / (t[1] - t[0]));
+            if (scheme_ == 3 && 3 < D.tend && 3 < int(t.size())) {
+                const double dt = t[3] - t[1], dt0 = t[3] - t[2];
+                out.push_back((dt + dt0) / (dt * dt0));
+            }
+            continue;
+        }
+        const int it = D.tbeg;
+        if (it >= D.tend || D.npre1 < 0 || D.npre2 < 0) continue;
+        if (scheme_ == 2) {
+            out.push_back(1.0 / (t[it] - t[D.npre1]));
+        } else {
+            const double dt = t[it] - t[D.npre2], dt0 = t[it] - t[D.npre1];
+            out.push_back((dt + dt0) / (dt * dt0));
+        }
+    }
+    return out;
+}
+
+// Starts the factorization of the next matrix of the schedule into the spare factor, on its
+// own stream, once the steps queued so far (which read the spare's previous factor) are done.
+void CalcCore::launch_prefetch() {
+    if (!prefetch_ || pending_ || fac_next_ >= fac_mults_.size()) return;
+    c64b_->set_tuning(c64_->tuning());
+    EMTD_HIP(hipEventRecord(swap_ev_, stream_));
+    EMTD_HIP(hipStreamWaitEvent(fstream_, swap_ev_, 0));
+    c64b_->set_stream(fstream_);
+    pending_mult_ = fac_mults_[fac_next_];
+    c64b_->factorize_system_async(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p,
+                                  pending_mult_);
+    pending_ = true;
+}
+
 void CalcCore::factorize(double mult) {
     const double t0 = wall_seconds();
     cur_mult_ = mult;
     int bad = -1;
     use64_ = (precision_ == EMTD_FP64);
+    if (prefetch_) {
+        const bool expected = fac_next_ < fac_mults_.size() && fac_mults_[fac_next_] == mult;
+        if (pending_ && expected && pending_mult_ == mult) {
+            // the look-ahead factor of this matrix: the steps wait for it on the device
+            EMTD_HIP(hipStreamWaitEvent(stream_, c64b_->factorization_done(), 0));
+            bad = c64b_->finish_factorization();
+            factor_gpu_seconds_ += c64b_->last_factor_seconds();
+            std::swap(c64_, c64b_);
+            c64_->set_stream(stream_);
+            c64b_->set_stream(fstream_);
+            pending_ = false;
+            ++prefetch_hits_;
+        } else {
+            if (pending_) {  // not the matrix asked for: let it finish and drop it
+                (void)c64b_->finish_factorization();
+                pending_ = false;
+                ++prefetch_misses_;
+            }
+            c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, mult);
+            bad = c64_->check_info();
+            factor_gpu_seconds_ += c64_->last_factor_seconds();
+        }
+        if (bad >= 0) throw Error(EMTD_ERR_NOTSPD, "non-positive pivot at permuted index " + std::to_string(bad));
+        if (expected) ++fac_next_;
+        else prefetch_ = false;  // the schedule does not match the time loop: no more look-ahead
+        factor_seconds_ += wall_seconds() - t0;
+        ++n_factorizations_;
+        if (autotune_ && !tuned_) run_autotune(mult);
+        tuning_used_ = c64_->tuning();
+        launch_prefetch();
+        return;
+    }
     // A smaller mult (a longer step) weakens the mass term that keeps the near-null space of
     // the curl-curl operator (gradients in the air) positive definite, so once the FP32
     // factorization has failed at some mult it is not tried again at a smaller or equal one.
@@ -613,6 +698,8 @@ void CalcCore::begin() {
     L_.uj2 = bufC_.p;
     vpre1_.assign(M_.times.size(), -1);
     vpre2_.assign(M_.times.size(), -2);
+    fac_mults_ = factorization_mults();
+    fac_next_ = 0;
     begun_ = true;
 }
 
@@ -775,6 +862,7 @@ int CalcCore::advance(int nsteps) {
     while ((nsteps < 0 || done < nsteps) && step()) ++done;
     EMTD_HIP(hipStreamSynchronize(stream_));
     for (const std::string& serr : {c64_ ? c64_->sweep_error() : std::string(),
+                                    c64b_ ? c64b_->sweep_error(stream_) : std::string(),
                                     c32_ ? c32_->sweep_error() : std::string()})
         if (!serr.empty()) throw Error(EMTD_ERR_ERROR, serr);
     step_seconds_ += (wall_seconds() - t0) - (factor_seconds_ - f0);
@@ -886,6 +974,7 @@ void CalcCore::info(emtd_calc_info* inf) const {
     inf->tuning[3] = tuning_used_.split_level_fronts;
     inf->tuning[4] = tuning_used_.small_front_sweeps;
     inf->tune_seconds = tune_seconds_;
+    inf->lookahead_factorizations = prefetch_hits_;
     inf->factorizations = n_factorizations_;
     inf->factor_gpu_seconds = factor_gpu_seconds_;
     inf->solves = n_solves_;

@@ -59,6 +59,8 @@ class CalcCore {
     bool stop_requested() const;
     void factorize(double mult);
     void run_autotune(double mult);
+    std::vector<double> factorization_mults() const;
+    void launch_prefetch();
     void solve(const double* b, double* x, const double* predictor);
     void raw_solve(const double* b, double* x);
     void refine(const double* b, double* x, const double* predictor);
@@ -83,6 +85,16 @@ class CalcCore {
     std::unique_ptr<Symbolic> S_;
     DevBuf<int64_t> dest_off_, dest_diag_;
     std::unique_ptr<GpuCholesky<double>> c64_;
+    // FP64 path: the next decade's matrix (known from the time grid) is factored into a second
+    // factor on its own stream while the current decade's steps run
+    std::unique_ptr<GpuCholesky<double>> c64b_;
+    bool prefetch_ = false, pending_ = false;
+    double pending_mult_ = 0;
+    std::vector<double> fac_mults_;
+    size_t fac_next_ = 0;
+    hipStream_t fstream_ = nullptr;
+    hipEvent_t swap_ev_ = nullptr;
+    int64_t prefetch_hits_ = 0, prefetch_misses_ = 0;
     std::unique_ptr<GpuCholesky<float>> c32_;
     DevBuf<int> perm_;
     DevBuf<int64_t> crp_;

@@ -1207,6 +1207,11 @@ GpuCholesky<T>::~GpuCholesky() {
         (void)hipEventDestroy(ev_main_);
         (void)hipEventDestroy(ev_side_);
     }
+    for (hipEvent_t e : fac_ev_)
+        if (e) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
 }
 
 template <typename T>
@@ -1312,13 +1317,14 @@ void GpuCholesky<T>::numeric() {
 }
 
 template <typename T>
-void GpuCholesky<T>::factorize_system(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low,
-                                      int n, const double* d_b_ggl, const double* d_b_di, const double* d_c_ggl,
-                                      const double* d_c_di, double mult) {
-    hipEvent_t e0, e1;
-    EMTD_HIP(hipEventCreate(&e0));
-    EMTD_HIP(hipEventCreate(&e1));
-    EMTD_HIP(hipEventRecord(e0, stream_));
+void GpuCholesky<T>::factorize_system_async(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low,
+                                            int n, const double* d_b_ggl, const double* d_b_di,
+                                            const double* d_c_ggl, const double* d_c_di, double mult) {
+    if (!fac_ev_[0]) {
+        EMTD_HIP(hipEventCreate(&fac_ev_[0]));
+        EMTD_HIP(hipEventCreate(&fac_ev_[1]));
+    }
+    EMTD_HIP(hipEventRecord(fac_ev_[0], stream_));
     const int init = INT_MAX;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, stream_, info_.p, int64_t(1), init);
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(S_.lsize, 256)), dim3(256), 0, stream_, L_.p, S_.lsize, T(0));
@@ -1329,13 +1335,24 @@ void GpuCholesky<T>::factorize_system(const int64_t* d_dest_off, const int64_t* 
                        d_c_di, n, mult, L_.p);
     numeric();
     invert_diagonal_blocks();
-    EMTD_HIP(hipEventRecord(e1, stream_));
-    EMTD_HIP(hipEventSynchronize(e1));
+    EMTD_HIP(hipEventRecord(fac_ev_[1], stream_));
+}
+
+template <typename T>
+int GpuCholesky<T>::finish_factorization() {
+    EMTD_HIP(hipEventSynchronize(fac_ev_[1]));
     float ms = 0;
-    EMTD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    EMTD_HIP(hipEventElapsedTime(&ms, fac_ev_[0], fac_ev_[1]));
     factor_seconds_ = ms * 1e-3;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    return check_info();
+}
+
+template <typename T>
+void GpuCholesky<T>::factorize_system(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low,
+                                      int n, const double* d_b_ggl, const double* d_b_di, const double* d_c_ggl,
+                                      const double* d_c_di, double mult) {
+    factorize_system_async(d_dest_off, d_dest_diag, nnz_low, n, d_b_ggl, d_b_di, d_c_ggl, d_c_di, mult);
+    (void)finish_factorization();
 }
 
 template <typename T>
@@ -1527,11 +1544,12 @@ void GpuCholesky<T>::invert_diagonal_blocks() {
 }
 
 template <typename T>
-std::string GpuCholesky<T>::sweep_error() {
+std::string GpuCholesky<T>::sweep_error(hipStream_t st) {
     if (!big_ready_ || !diag_.p) return {};
+    if (!st) st = stream_;
     SweepDiag h{};
-    EMTD_HIP(hipMemcpyAsync(&h, diag_.p, sizeof(SweepDiag), hipMemcpyDeviceToHost, stream_));
-    EMTD_HIP(hipStreamSynchronize(stream_));
+    EMTD_HIP(hipMemcpyAsync(&h, diag_.p, sizeof(SweepDiag), hipMemcpyDeviceToHost, st));
+    EMTD_HIP(hipStreamSynchronize(st));
     if (h.err == 0) return {};
     static const char* kinds[] = {"?", "forward chain", "backward chain (earlier block)", "backward chain (next block)"};
     char buf[512];

@@ -66,6 +66,16 @@ class GpuCholesky {
     void factorize_system(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
                           const double* d_b_ggl, const double* d_b_di, const double* d_c_ggl,
                           const double* d_c_di, double mult);
+    // The same without waiting: the work is queued on the solver's stream; finish_factorization()
+    // waits for it (factorization_done() is recorded at its end) and returns check_info().
+    void factorize_system_async(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
+                                const double* d_b_ggl, const double* d_b_di, const double* d_c_ggl,
+                                const double* d_c_di, double mult);
+    int finish_factorization();
+    hipEvent_t factorization_done() const { return fac_ev_[1]; }
+    // Stream of the solver's factorization and solves (a factor can be built on one stream and
+    // used on another).
+    void set_stream(hipStream_t s) { stream_ = s; }
     // Numeric factorization of a given lower RSF matrix (ggl, di) on the device.
     void factorize_values(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
                           const double* d_ggl, const double* d_di);
@@ -74,8 +84,9 @@ class GpuCholesky {
 
     // Returns -1 or the first (permuted) pivot that was not positive; synchronizes.
     int check_info();
-    // Empty, or the description of the split-sweep wait that hit its limit (synchronizes).
-    std::string sweep_error();
+    // Empty, or the description of the split-sweep wait that hit its limit (synchronizes the
+    // given stream, by default the solver's).
+    std::string sweep_error(hipStream_t s = nullptr);
     double last_factor_seconds() const { return factor_seconds_; }
     // Launch parameters. A change of the sweep parameters rebuilds the sweep plan and the
     // inverted pivot blocks (from the current factor) before the next solve.
@@ -128,6 +139,7 @@ class GpuCholesky {
     int n_inv_items_ = 0;
     int max_blocks_ = 1;  // most 64-wide pivot blocks in one split-sweep level
     double factor_seconds_ = 0;
+    hipEvent_t fac_ev_[2] = {nullptr, nullptr};
 };
 
 }  // namespace emtd

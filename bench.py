@@ -213,7 +213,8 @@ def timed_leg(emtd, d, spec, precision, args):
                    "factor_tflops": fac_tflops, "solve_ms": round(solve_s * 1e3, 4),
                    "setup_s": round(ci.setup_seconds, 2),
                    "refine_iterations_per_step": round((inf1.refine_iterations - inf0.refine_iterations) / K, 4),
-                   "fallbacks": int(inf1.fallbacks), "final_relative_residual": inf1.last_residual},
+                   "fallbacks": int(inf1.fallbacks), "final_relative_residual": inf1.last_residual,
+                   "lookahead_factorizations": int(inf1.lookahead_factorizations - inf0.lookahead_factorizations)},
         "roofline": None,
     }
     # the two per-step hot paths; "roofline" is the one with the larger share of the timed

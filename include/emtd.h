@@ -153,6 +153,9 @@ typedef struct emtd_calc_info {
                                     (target columns), look-ahead LDS pad, split-sweep threshold
                                     (fronts per level), one-wavefront-per-front sweeps (0/1) */
     double tune_seconds;         /* time spent by the autotuner (environment EMTD_AUTOTUNE=1) */
+    int64_t lookahead_factorizations; /* factorizations done ahead, overlapped with the previous
+                                         decade's steps (FP64 path; EMTD_PREFETCH_FACTOR=0 turns
+                                         the look-ahead off) */
 } emtd_calc_info;
 int emtd_calc_get_info(const emtd_calc* c, emtd_calc_info* info);
 void emtd_calc_destroy(emtd_calc* c);
