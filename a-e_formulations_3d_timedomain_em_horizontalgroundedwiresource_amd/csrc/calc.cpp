@@ -34,7 +34,7 @@ using namespace emtd;
 namespace emtd {
 
 CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, int device,
-                   const std::function<void(int64_t)>& admit)
+                   const std::function<void(int64_t)>& admit, bool lookahead)
     : dir_(dir), device_(device), precision_(precision) {
     const double t0 = wall_seconds();
     select_device(device);
@@ -43,7 +43,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
     if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
     if (const char* e = std::getenv("EMTD_AUTOTUNE")) autotune_ = std::atoi(e) != 0;
-    prefetch_ = precision_ == EMTD_FP64;
+    prefetch_ = lookahead && precision_ == EMTD_FP64;
     if (const char* e = std::getenv("EMTD_PREFETCH_FACTOR")) prefetch_ = prefetch_ && std::atoi(e) != 0;
     n_ = M_.n;
     npls_ = M_.npls;

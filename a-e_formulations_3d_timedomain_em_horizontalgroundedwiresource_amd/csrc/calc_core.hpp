@@ -22,8 +22,10 @@ class CalcCore {
   public:
     // admit (optional) is called with the estimated device footprint after the host-side
     // analysis and before any device allocation; it may block (admission control) or throw.
+    // lookahead: factor the next decade's matrix on a second stream during the current
+    // decade's steps (FP64; EMTD_PREFETCH_FACTOR=0 turns it off).
     CalcCore(const std::string& dir, int form, int scheme, int precision, int device,
-             const std::function<void(int64_t)>& admit = {});
+             const std::function<void(int64_t)>& admit = {}, bool lookahead = true);
     ~CalcCore();
     void run();
     int advance(int nsteps);  // runs up to nsteps steps (all when negative); synchronizes

@@ -198,8 +198,10 @@ class Server {
             int n_edges = 0;
             int64_t steps = 0;
             try {
+                // no look-ahead factorization here: the other requests in flight already keep
+                // the GPU busy while one refactors, and it would double the factor memory
                 auto core = std::make_unique<CalcCore>(job.dir, job.form, job.scheme, job.precision, device_,
-                                                       [&](int64_t bytes) { admit(id, bytes); });
+                                                       [&](int64_t bytes) { admit(id, bytes); }, false);
                 core->run();
                 core->write_outputs();
                 n_edges = core->n_edges();

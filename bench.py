@@ -8,7 +8,11 @@ RHS (mass-matrix SpMV + source), forward/backward triangular sweeps with the GPU
 receiver evaluation; the first step of each decade also refactorizes the system matrix.
 
 Prints ONE JSON line. ``value`` = timed steps / wall time of the timed region (inputs and
-factor storage already resident in HBM) of the FP64 path. Beside it, the same model runs with
+factor storage already resident in HBM) of the FP64 path, where the next decade's matrix is
+factored on a second stream while the current decade's steps run (look-ahead). The kernel
+rooflines come from the same workload timed again without look-ahead, so that every kernel
+runs alone on the GPU (``roofline``, ``roofline_other``; the overlapped durations are in
+``roofline_lookahead``). Beside it, the same model runs with
 the FP32 factor + FP64 iterative refinement path (``fp32_ir``: its steps/s, speedup over FP64,
 refinement iterations per step, FP64 fallbacks, final residual, and the largest receiver
 difference from the FP64 run relative to the FP64 values' largest magnitude). Two per-step hot paths are measured with HIP events
@@ -147,7 +151,7 @@ def measured_traffic(args, cells, what):
     FETCH_SIZE/WRITE_SIZE on this workload (the summaries under profiles/ give the commands
     and the copy-kernel calibration), or None when the run is not the profiled workload."""
     if args.config != 3 or cells is not None or args.precision != "fp64":
-        return None
+        return None   # the counters were taken on the FP64 workload with every kernel alone
     name = {"solve": "sweeps_config3_traffic.md", "factor": "factorization_config3_traffic.md"}[what]
     key = {"solve": "traffic_bytes_per_solve:", "factor": "traffic_bytes_per_factorization:"}[what]
     try:
@@ -160,14 +164,19 @@ def measured_traffic(args, cells, what):
     return None
 
 
-def timed_leg(emtd, d, spec, precision, args):
+def timed_leg(emtd, d, spec, precision, args, lookahead=True):
     """Runs the whole time loop of the model in ``d`` at one precision: ``warmup`` untimed
-    steps, then ``steps`` timed steps. Returns the JSON line's fields and the receiver values."""
-    import numpy as np
+    steps, then ``steps`` timed steps. Returns the JSON line's fields and the receiver values.
+    ``lookahead`` False turns off the factorization of the next decade's matrix during the
+    current decade's steps (FP64), so every kernel runs alone on the GPU."""
     prec = emtd.FP64 if precision == "fp64" else emtd.FP32_IR
     cells = tuple(int(v) for v in args.cells.split("x")) if args.cells else None
     t0 = time.time()
-    c = emtd.Calc(d, 0 if spec.formulation == "A" else 1, spec.scheme, prec)
+    os.environ["EMTD_PREFETCH_FACTOR"] = "1" if lookahead else "0"
+    try:
+        c = emtd.Calc(d, 0 if spec.formulation == "A" else 1, spec.scheme, prec)
+    finally:
+        os.environ.pop("EMTD_PREFETCH_FACTOR", None)
     ci = c.info()
     log(f"[bench] {precision} setup {time.time() - t0:.1f} s: n={ci.n_edges} factored={ci.n_factored} "
         f"fronts={ci.n_fronts} levels={ci.n_levels} nnzL={ci.nnz_l:.4g} flops={ci.factor_flops:.4g}")
@@ -197,7 +206,8 @@ def timed_leg(emtd, d, spec, precision, args):
     nfac = inf1.factorizations - inf0.factorizations
     fac_gpu = (inf1.factor_gpu_seconds - inf0.factor_gpu_seconds) / max(nfac, 1)
     fac_tflops = ci.factor_flops / fac_gpu / 1e12 if nfac else None
-    log(f"[bench] {precision}: {K} steps in {elapsed:.3f} s -> {value:.2f} steps/s; solve {solve_s * 1e3:.3f} ms "
+    log(f"[bench] {precision}{'' if lookahead else ' (no look-ahead)'}: {K} steps in {elapsed:.3f} s -> "
+        f"{value:.2f} steps/s; solve {solve_s * 1e3:.3f} ms "
         f"({achieved:.0f} GB/s); {nfac} factorizations {fac_gpu * 1e3:.1f} ms each ({fac_tflops} TF/s)")
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "timesteps/s", "n_gpus": 1, "steps": K,
@@ -220,7 +230,7 @@ def timed_leg(emtd, d, spec, precision, args):
     # the two per-step hot paths; "roofline" is the one with the larger share of the timed
     # region (the other is reported beside it)
     peak_tf = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
-    prof = argparse.Namespace(config=args.config, precision=precision)
+    prof = argparse.Namespace(config=args.config, precision=precision if not lookahead else "overlapped")
     r_solve = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(prof, cells, "solve"),
                "kernel": ("one triangular solve (forward + backward sweep kernels); bytes = 2 sweeps x %d B "
@@ -309,16 +319,33 @@ def main():
     d, spec, info = make_workload(args.config, cells)
     log(f"[bench] model: {info} in {time.time() - t0:.1f} s")
     try:
-        legs = [args.precision]
-        if args.compare and args.precision == "fp64":
-            legs.append("fp32")
-        res = {p: timed_leg(emtd, d, spec, p, args) for p in legs}
+        res = {args.precision: timed_leg(emtd, d, spec, args.precision, args)}
+        if args.precision == "fp64":
+            # the same workload with every kernel alone on the GPU: the kernel rooflines
+            res["fp64_alone"] = timed_leg(emtd, d, spec, "fp64", args, lookahead=False)
+            if args.compare:
+                res["fp32"] = timed_leg(emtd, d, spec, "fp32", args)
     finally:
         shutil.rmtree(d, ignore_errors=True)
     main_leg = res[args.precision]
     out = main_leg["out"]
+    if "fp64_alone" in res:
+        alone = res["fp64_alone"]["out"]
+        out["roofline_lookahead"] = {"factorization": out["roofline"] if out["roofline"]["bound"] == "mfma"
+                                     else out["roofline_other"],
+                                     "solve": out["roofline"] if out["roofline"]["bound"] == "hbm"
+                                     else out["roofline_other"],
+                                     "note": "durations in the timed region above, where the next decade's "
+                                             "factorization overlaps the current decade's steps"}
+        out["roofline"], out["roofline_other"] = alone["roofline"], alone["roofline_other"]
+        for r in (out["roofline"], out["roofline_other"]):
+            if r:
+                r["measured_in"] = ("the same workload timed without look-ahead (each kernel alone on the GPU): "
+                                    f"{alone['value']} steps/s")
+        out["value_without_lookahead"] = alone["value"]
+        out["lookahead_speedup"] = round(out["value"] / alone["value"], 4)
     if "fp32" in res and args.precision == "fp64":
-        f32, f64 = res["fp32"], res["fp64"]
+        f32, f64 = res["fp32"], res["fp64_alone"]
         errs = {}
         for kind in ("B", "E"):
             a, b = f32["receivers"][kind], f64["receivers"][kind]
@@ -326,7 +353,8 @@ def main():
             errs[kind] = float(np.abs(a - b).max() / scale) if scale > 0 else None
         out["fp32_ir"] = {
             "timesteps_per_s": f32["out"]["value"], "ms_per_step": f32["out"]["ms_per_step"],
-            "speedup_vs_fp64": round(f32["out"]["value"] / f64["out"]["value"], 4),
+            "speedup_vs_fp64": round(f32["out"]["value"] / out["value"], 4),
+            "speedup_vs_fp64_without_lookahead": round(f32["out"]["value"] / f64["out"]["value"], 4),
             "max_rel_receiver_err_vs_fp64": errs,
             "meets_1e-5": all(e is not None and e <= 1e-5 for e in errs.values()),
             **{k: f32["out"]["config"][k] for k in ("timed_factorizations", "factor_ms", "factor_tflops",
