@@ -45,6 +45,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     if (const char* e = std::getenv("EMTD_AUTOTUNE")) autotune_ = std::atoi(e) != 0;
     prefetch_ = lookahead && precision_ == EMTD_FP64;
     if (const char* e = std::getenv("EMTD_PREFETCH_FACTOR")) prefetch_ = prefetch_ && std::atoi(e) != 0;
+    if (const char* e = std::getenv("EMTD_LOOKAHEAD_UPDATE_PAD")) lookahead_update_pad_ = std::max(0, std::atoi(e));
     n_ = M_.n;
     npls_ = M_.npls;
     // host: pattern, ordering and symbolic analysis (Dirichlet edges are singletons)
@@ -90,6 +91,8 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
             EMTD_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
             EMTD_HIP(hipEventCreateWithFlags(&swap_ev_, hipEventDisableTiming));
             c64b_ = std::make_unique<GpuCholesky<double>>(*S_, fstream_);
+            c64b_->prepare_sweeps();
+            c64_->prepare_sweeps();
         }
     } else
         c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
@@ -258,6 +261,7 @@ CalcCore::~CalcCore() {
         c64b_.reset();
         c64_.reset();
         c32_.reset();
+        spare_layers_.clear();
         if (swap_ev_) (void)hipEventDestroy(swap_ev_);
         if (fstream_) (void)hipStreamDestroy(fstream_);
         stored_.clear();
@@ -368,11 +372,16 @@ std::vector<double> CalcCore::factorization_mults() const {
 }
 
 // Starts the factorization of the next matrix of the schedule into the spare factor, on its
-// own stream, once the steps queued so far (which read the spare's previous factor) are done.
+// own stream, once the steps queued before the last factorization (which read the spare's
+// previous factor; swap_ev_ marks them) are done. Called after the first step of a decade has
+// been queued, so that the GPU has step work while the host queues the factorization.
 void CalcCore::launch_prefetch() {
+    if (!prefetch_due_) return;
+    prefetch_due_ = false;
     if (!prefetch_ || pending_ || fac_next_ >= fac_mults_.size()) return;
-    c64b_->set_tuning(c64_->tuning());
-    EMTD_HIP(hipEventRecord(swap_ev_, stream_));
+    Tuning t = c64_->tuning();
+    t.update_lds_pad = lookahead_update_pad_;
+    c64b_->set_tuning(t);
     EMTD_HIP(hipStreamWaitEvent(fstream_, swap_ev_, 0));
     c64b_->set_stream(fstream_);
     pending_mult_ = fac_mults_[fac_next_];
@@ -415,7 +424,10 @@ void CalcCore::factorize(double mult) {
         ++n_factorizations_;
         if (autotune_ && !tuned_) run_autotune(mult);
         tuning_used_ = c64_->tuning();
-        launch_prefetch();
+        if (fac_next_ < fac_mults_.size() && prefetch_) {
+            EMTD_HIP(hipEventRecord(swap_ev_, stream_));
+            prefetch_due_ = true;  // launched by the step once its own work is queued
+        }
         return;
     }
     // A smaller mult (a longer step) weakens the mass term that keeps the near-null space of
@@ -663,7 +675,14 @@ void CalcCore::output_layer(int layer, const double* u) {
 
 double* CalcCore::keep_layer(int layer, const double* u) {
     auto it = stored_.find(layer);
-    if (it == stored_.end()) it = stored_.emplace(layer, DevBuf<double>(size_t(n_) * npls_)).first;
+    if (it == stored_.end()) {
+        if (!spare_layers_.empty()) {
+            it = stored_.emplace(layer, std::move(spare_layers_.back())).first;
+            spare_layers_.pop_back();
+        } else {
+            it = stored_.emplace(layer, DevBuf<double>(size_t(n_) * npls_)).first;
+        }
+    }
     EMTD_HIP(hipMemcpyAsync(it->second.p, u, size_t(n_) * npls_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     if (snapshots_) write_snapshot(layer, u);
     return it->second.p;
@@ -742,6 +761,7 @@ void CalcCore::two_layer_step() {
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj, L_.uj, 1.0 / dtl, 0.0,
                src_.n ? src_.p : nullptr, source_scale(it, dtl), cdv_.p, en, rhs_.p);
     solve(rhs_.p, L_.uj2, L_.uj);
+    launch_prefetch();
     std::swap(L_.uj, L_.uj2);
     if (it >= D.nlst) keep_layer(it, L_.uj);
     vpre1_[it] = it - 1;
@@ -769,8 +789,12 @@ void CalcCore::end_decade() {
         bool needed = false;
         for (size_t k2 = L_.k + 1; k2 < plan_.size(); ++k2)
             if (itS->first == plan_[k2].npre1 || itS->first == plan_[k2].npre2) needed = true;
-        if (!needed) itS = stored_.erase(itS);
-        else ++itS;
+        if (!needed) {
+            spare_layers_.push_back(std::move(itS->second));
+            itS = stored_.erase(itS);
+        } else {
+            ++itS;
+        }
     }
     L_.in_decade = false;
     ++L_.k;
@@ -839,6 +863,7 @@ bool CalcCore::step() {
     launch_rhs(stream_, nn_, n_, npls_, crp_.p, cci_.p, ccv_.p, L_.uj1, L_.uj2, L_.mult1, L_.mult2,
                src_.n ? src_.p : nullptr, source_scale(it, -1.0), cdv_.p, en, rhs_.p);
     solve(rhs_.p, L_.uj, L_.uj1);
+    launch_prefetch();
     if (it >= D.nlst) keep_layer(it, L_.uj);
     if (L_.tn1 < last_) output_layer(L_.tn1, L_.uj1);
     if (L_.tn == last_) output_layer(L_.tn, L_.uj);

@@ -90,13 +90,14 @@ class CalcCore {
     // FP64 path: the next decade's matrix (known from the time grid) is factored into a second
     // factor on its own stream while the current decade's steps run
     std::unique_ptr<GpuCholesky<double>> c64b_;
-    bool prefetch_ = false, pending_ = false;
+    bool prefetch_ = false, pending_ = false, prefetch_due_ = false;
     double pending_mult_ = 0;
     std::vector<double> fac_mults_;
     size_t fac_next_ = 0;
     hipStream_t fstream_ = nullptr;
     hipEvent_t swap_ev_ = nullptr;
     int64_t prefetch_hits_ = 0, prefetch_misses_ = 0;
+    int lookahead_update_pad_ = 0;  // update_lds_pad of the look-ahead factorizations
     std::unique_ptr<GpuCholesky<float>> c32_;
     DevBuf<int> perm_;
     DevBuf<int64_t> crp_;
@@ -126,6 +127,7 @@ class CalcCore {
     std::vector<Decade> plan_;
     std::vector<int> vpre1_, vpre2_;
     std::map<int, DevBuf<double>> stored_;
+    std::vector<DevBuf<double>> spare_layers_;  // freed restart layers, reused (no hipFree in the loop)
     DevBuf<double> last_field_;
     bool ran_ = false, begun_ = false;
     LoopState L_;

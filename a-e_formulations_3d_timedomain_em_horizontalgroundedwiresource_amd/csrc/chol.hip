@@ -1121,6 +1121,7 @@ void GpuCholesky<T>::set_tuning(const Tuning& t) {
     Tuning n = t;
     n.outer_block = std::max(NB, (n.outer_block / NB) * NB);
     n.lookahead_lds_pad = std::min(std::max(0, n.lookahead_lds_pad), 60 * 1024);
+    n.update_lds_pad = std::min(std::max(0, n.update_lds_pad), 60 * 1024);
     if (n.split_level_fronts != tune_.split_level_fronts || n.small_front_sweeps != tune_.small_front_sweeps) {
         big_ready_ = false;
         inv_stale_ = true;
@@ -1221,7 +1222,8 @@ void GpuCholesky<T>::update(hipStream_t st, const int* fz, int cnt, int k0, int 
     constexpr int KC = 128 / int(sizeof(T));
     if (cols >= tune_.wide_tile_cols) {
         const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
-        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), lds_pad, st, fronts_.p, fz, k0,
+        const size_t pad = std::max(lds_pad, size_t(tune_.update_lds_pad));
+        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), pad, st, fronts_.p, fz, k0,
                            k1, cmin, cmax, L_.p, U_.p);
     } else {
         const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
@@ -1530,6 +1532,15 @@ void GpuCholesky<T>::prepare_inverses() {
     ioff_ = to_device(ioff, stream_);
     inv_.alloc(size_t(std::max<int64_t>(o, 1)));
     n_inv_items_ = int(items.size());
+}
+
+template <typename T>
+void GpuCholesky<T>::prepare_sweeps() {
+    if (!big_ready_) prepare_big_levels();
+    if (!inv_.p || inv_stale_) {
+        prepare_inverses();
+        inv_stale_ = false;
+    }
 }
 
 template <typename T>

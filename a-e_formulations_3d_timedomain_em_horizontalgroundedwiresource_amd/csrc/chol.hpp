@@ -41,10 +41,12 @@ struct Tuning {
     int lookahead_lds_pad = 0;      // extra LDS bytes of the look-ahead bulk update
     int split_level_fronts = 1024;  // levels with fewer fronts use the split sweeps
     int small_front_sweeps = 1;     // one wavefront per front where every front has <= 64 pivots
+    int update_lds_pad = 0;         // extra LDS bytes of every 128x128-tile update launch (room on
+                                    // each CU for another stream's kernels during a look-ahead)
     bool operator==(const Tuning& o) const {
         return outer_block == o.outer_block && wide_tile_cols == o.wide_tile_cols &&
                lookahead_lds_pad == o.lookahead_lds_pad && split_level_fronts == o.split_level_fronts &&
-               small_front_sweeps == o.small_front_sweeps;
+               small_front_sweeps == o.small_front_sweeps && update_lds_pad == o.update_lds_pad;
     }
 };
 
@@ -92,6 +94,9 @@ class GpuCholesky {
     // inverted pivot blocks (from the current factor) before the next solve.
     const Tuning& tuning() const { return tune_; }
     void set_tuning(const Tuning& t);
+    // Builds the sweep plan and the storage of the inverted pivot blocks now (host work and
+    // allocations that would otherwise happen at the first factorization or solve).
+    void prepare_sweeps();
 
     const Symbolic& sym() const { return S_; }
     hipStream_t stream() const { return stream_; }

@@ -164,7 +164,7 @@ def measured_traffic(args, cells, what):
     return None
 
 
-def timed_leg(emtd, d, spec, precision, args, lookahead=True):
+def timed_leg(emtd, d, spec, precision, args, lookahead=True, finish=False):
     """Runs the whole time loop of the model in ``d`` at one precision: ``warmup`` untimed
     steps, then ``steps`` timed steps. Returns the JSON line's fields and the receiver values.
     ``lookahead`` False turns off the factorization of the next decade's matrix during the
@@ -194,8 +194,10 @@ def timed_leg(emtd, d, spec, precision, args, lookahead=True):
     c.set_timing(False)
     inf1 = c.info()
     assert got == K, (got, K)
-    c.step(-1)
-    receivers = {"B": c.receiver_values(0), "E": c.receiver_values(1)}
+    receivers = None
+    if finish:  # the rest of the time loop, for the receiver comparison between precisions
+        c.step(-1)
+        receivers = {"B": c.receiver_values(0), "E": c.receiver_values(1)}
     elapsed = t2 - t1
     value = K / elapsed
     nsolve = inf1.timed_solves
@@ -269,6 +271,9 @@ def main():
     ap.add_argument("--cells", type=str, default=None, help="override cells, e.g. 40x40x30")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lookahead", choices=["both", "on", "off"], default="both",
+                    help="FP64 legs: with look-ahead factorization (the value), without (the kernel "
+                         "rooflines), or both")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the FP32+IR leg that runs beside the FP64 one on the same model")
     ap.add_argument("--no-serving", action="store_true", help="skip the short serving sample")
@@ -319,17 +324,20 @@ def main():
     d, spec, info = make_workload(args.config, cells)
     log(f"[bench] model: {info} in {time.time() - t0:.1f} s")
     try:
-        res = {args.precision: timed_leg(emtd, d, spec, args.precision, args)}
-        if args.precision == "fp64":
+        res = {}
+        if args.lookahead in ("both", "on") or args.precision != "fp64":
+            res[args.precision] = timed_leg(emtd, d, spec, args.precision, args)
+        compare = args.precision == "fp64" and args.compare and args.lookahead in ("both", "off")
+        if args.precision == "fp64" and args.lookahead in ("both", "off"):
             # the same workload with every kernel alone on the GPU: the kernel rooflines
-            res["fp64_alone"] = timed_leg(emtd, d, spec, "fp64", args, lookahead=False)
-            if args.compare:
-                res["fp32"] = timed_leg(emtd, d, spec, "fp32", args)
+            res["fp64_alone"] = timed_leg(emtd, d, spec, "fp64", args, lookahead=False, finish=compare)
+        if compare:
+            res["fp32"] = timed_leg(emtd, d, spec, "fp32", args, finish=True)
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    main_leg = res[args.precision]
+    main_leg = res[args.precision] if args.precision in res else res["fp64_alone"]
     out = main_leg["out"]
-    if "fp64_alone" in res:
+    if "fp64_alone" in res and args.precision in res:
         alone = res["fp64_alone"]["out"]
         out["roofline_lookahead"] = {"factorization": out["roofline"] if out["roofline"]["bound"] == "mfma"
                                      else out["roofline_other"],
@@ -344,7 +352,7 @@ def main():
                                     f"{alone['value']} steps/s")
         out["value_without_lookahead"] = alone["value"]
         out["lookahead_speedup"] = round(out["value"] / alone["value"], 4)
-    if "fp32" in res and args.precision == "fp64":
+    if "fp32" in res and args.precision == "fp64" and "fp64_alone" in res:
         f32, f64 = res["fp32"], res["fp64_alone"]
         errs = {}
         for kind in ("B", "E"):
