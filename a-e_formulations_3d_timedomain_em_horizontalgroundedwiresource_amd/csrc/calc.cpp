@@ -14,6 +14,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,21 @@
 using namespace emtd;
 
 namespace emtd {
+
+void keep_pool_memory(int device) {
+    static std::mutex mu;
+    static std::vector<int> done;
+    int dev = device;
+    if (dev < 0) EMTD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    if (std::find(done.begin(), done.end(), dev) != done.end()) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    done.push_back(dev);
+}
 
 CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, int device,
                    const std::function<void(int64_t)>& admit, bool lookahead)
@@ -61,7 +77,10 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     // device memory is claimed only after the caller admitted the footprint
     footprint_ = estimate_footprint();
     if (admit) admit(footprint_);
-    EMTD_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    EMTD_HIP(hipStreamCreateWithFlags(&sown_.s, hipStreamNonBlocking));
+    stream_ = sown_.s;
+    keep_pool_memory(device);
+    AllocScope alloc_scope(stream_);  // the model's buffers: stream-ordered on stream_
     // GPU assembly of B and C (original numbering)
     emtd_mesh mesh{M_.kuzlov, M_.kpar, n_, M_.nmat, M_.xyz.data(), M_.nver.data(), M_.ed.data(),
                    M_.edges.data(), M_.nvkat.data(), M_.sigma.data(), M_.mu.data()};
@@ -90,7 +109,8 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         if (prefetch_) {
             EMTD_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
             EMTD_HIP(hipEventCreateWithFlags(&swap_ev_, hipEventDisableTiming));
-            c64b_ = std::make_unique<GpuCholesky<double>>(*S_, fstream_);
+            // built on the step stream (its uploads), used on fstream_ after events
+            c64b_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
             c64b_->prepare_sweeps();
             c64_->prepare_sweeps();
         }
@@ -264,6 +284,7 @@ CalcCore::~CalcCore() {
         spare_layers_.clear();
         if (swap_ev_) (void)hipEventDestroy(swap_ev_);
         if (fstream_) (void)hipStreamDestroy(fstream_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
         stored_.clear();
         for (int k = 0; k < 2; ++k) {
             if (en_ev_[k]) {
@@ -273,7 +294,6 @@ CalcCore::~CalcCore() {
             if (en_stage_[k]) (void)hipHostFree(en_stage_[k]);
         }
         for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
-        if (stream_) (void)hipStreamDestroy(stream_);
     } catch (...) {
     }
 }
@@ -381,7 +401,14 @@ void CalcCore::launch_prefetch() {
     if (!prefetch_ || pending_ || fac_next_ >= fac_mults_.size()) return;
     Tuning t = c64_->tuning();
     t.update_lds_pad = lookahead_update_pad_;
-    c64b_->set_tuning(t);
+    if (!(t == c64b_->tuning())) {
+        // a new sweep plan may reallocate (stream-ordered on the step stream): build it there,
+        // and order the look-ahead after it
+        c64b_->set_tuning(t);
+        c64b_->set_stream(stream_);
+        c64b_->prepare_sweeps();
+        EMTD_HIP(hipEventRecord(swap_ev_, stream_));
+    }
     EMTD_HIP(hipStreamWaitEvent(fstream_, swap_ev_, 0));
     c64b_->set_stream(fstream_);
     pending_mult_ = fac_mults_[fac_next_];
@@ -881,6 +908,7 @@ bool CalcCore::step() {
 }
 
 int CalcCore::advance(int nsteps) {
+    AllocScope alloc_scope(stream_);
     const double t0 = wall_seconds();
     const double f0 = factor_seconds_;
     int done = 0;

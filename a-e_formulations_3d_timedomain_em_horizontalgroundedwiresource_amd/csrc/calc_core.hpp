@@ -74,6 +74,7 @@ class CalcCore {
     double* keep_layer(int layer, const double* u);
     void write_snapshot(int layer, const double* u) const;
 
+    OwnedStream sown_;  // first member: outlives every device buffer (their frees are ordered on it)
     std::string dir_;
     int device_ = -1, precision_ = 0, form_ = 0, scheme_ = 2;
     hipStream_t stream_ = nullptr;

@@ -28,18 +28,41 @@ void set_last_error(const std::string& s);
                                             " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
     } while (0)
 
+// Stream-ordered allocation. While an AllocScope for stream s is active on the calling thread,
+// DevBuf allocates with hipMallocAsync on s and frees with hipFreeAsync on s: neither waits for
+// other streams, whereas hipFree waits for all work on the device (which stalls every other
+// model in flight). The owner keeps s alive until its buffers are released and uses them only
+// on s or on streams ordered after s by events. Outside a scope: hipMalloc / hipFree.
+inline hipStream_t& alloc_stream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+    ~AllocScope() { alloc_stream() = prev; }
+    AllocScope(const AllocScope&) = delete;
+    AllocScope& operator=(const AllocScope&) = delete;
+};
+
 // Owning device allocation.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    hipStream_t s = nullptr;  // allocated with hipMallocAsync on s (freed on s)
+    bool async = false;
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s), async(o.async) { o.p = nullptr; o.n = 0; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; s = o.s; async = o.async;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
@@ -47,18 +70,27 @@ struct DevBuf {
         release();
         n = count;
         if (count) {
-            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            const hipStream_t st = alloc_stream();
+            hipError_t e = st ? hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), st)
+                              : hipMalloc(&p, count * sizeof(T));
             if (e != hipSuccess) {
                 p = nullptr;
                 n = 0;
-                throw Error(0x0001, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+                throw Error(0x0001, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
             }
+            s = st;
+            async = st != nullptr;
         }
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (async) (void)hipFreeAsync(p, s);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        async = false;
+        s = nullptr;
     }
     void upload(const T* h, size_t count, hipStream_t s = 0) {
         if (count > n) throw Error(0x0004, "upload larger than buffer");
@@ -79,6 +111,25 @@ DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s = 0) {
     b.upload(v.data(), v.size(), s);
     return b;
 }
+
+// A stream destroyed (after waiting for it) when its owner goes: declared before the owner's
+// device buffers, so that it outlives their stream-ordered frees.
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    OwnedStream() = default;
+    OwnedStream(const OwnedStream&) = delete;
+    OwnedStream& operator=(const OwnedStream&) = delete;
+    ~OwnedStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+// Keeps the memory that stream-ordered frees return to the device's default pool for reuse
+// (release threshold = unlimited), once per device.
+void keep_pool_memory(int device);
 
 inline double wall_seconds() {
     using namespace std::chrono;
