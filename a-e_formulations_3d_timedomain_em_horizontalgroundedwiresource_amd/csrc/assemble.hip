@@ -17,6 +17,10 @@
 // with M(s,t) = 1/3 if s==t else 1/6, D(s,t) = 1 if s==t else -1, L'(s) = -1 for s=0, +1.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
 #include "assemble.hpp"
 
 namespace emtd {
@@ -105,55 +109,75 @@ __global__ void k_assemble(int64_t nnz, int64_t nslots, const int64_t* __restric
 
 void build_contributions(int kpar, const int* ed, int n, const int* ig, const int* jg, std::vector<int64_t>& cptr,
                          std::vector<uint32_t>& codes) {
-    // slots: 0..nnz_low-1 lower entries, nnz_low..nnz_low+n-1 diagonal
+    // slots: 0..nnz_low-1 lower entries, nnz_low..nnz_low+n-1 diagonal. Built slot by slot: the
+    // elements that contribute to entry (i, j) are those holding both edges, in ascending element
+    // order (the order in which the reference's element loop adds them), each with the one local
+    // pair (a, b) that has ed[a] = i and ed[b] = j.
     const int64_t nnz = ig[n];
     const int64_t ns = nnz + n;
+    std::vector<int> eptr(n + 1, 0), eel(size_t(kpar) * 12);
+    for (int64_t q = 0; q < int64_t(kpar) * 12; ++q) eptr[ed[q] + 1]++;
+    for (int i = 0; i < n; ++i) eptr[i + 1] += eptr[i];
+    {
+        std::vector<int> fill(eptr.begin(), eptr.end() - 1);
+        for (int e = 0; e < kpar; ++e)
+            for (int a = 0; a < 12; ++a) eel[fill[ed[12 * e + a]]++] = e;
+    }
+    auto local = [&](int e, int edge) {
+        const int* E = ed + 12 * e;
+        int a = 0;
+        while (E[a] != edge) ++a;
+        return a;
+    };
+    // calls emit(e) for the common elements of edges i and j in ascending order
+    auto common = [&](int i, int j, auto&& emit) {
+        int p = eptr[i], q = eptr[j];
+        const int pe = eptr[i + 1], qe = eptr[j + 1];
+        while (p < pe && q < qe) {
+            if (eel[p] < eel[q]) ++p;
+            else if (eel[q] < eel[p]) ++q;
+            else { emit(eel[p]); ++p; ++q; }
+        }
+    };
     cptr.assign(ns + 1, 0);
-    std::vector<int64_t> slot(size_t(kpar) * 78);
-    int64_t w = 0;
-    for (int e = 0; e < kpar; ++e) {
-        const int* E = ed + 12 * e;
-        for (int a = 0; a < 12; ++a) {
-            const int ii = E[a];
-            for (int b = 0; b < 12; ++b) {
-                const int jj = E[b];
-                int64_t s = -1;
-                if (jj < ii) {
-                    const int* lo = jg + ig[ii];
-                    const int* hi = jg + ig[ii + 1];
-                    const int* it = std::lower_bound(lo, hi, jj);
-                    if (it == hi || *it != jj) throw Error(0x0004, "assembly: pattern lacks an element entry");
-                    s = it - jg;
-                } else if (jj == ii && a == b) {
-                    s = nnz + ii;
-                }
-                if (s >= 0) cptr[s + 1]++;
-            }
+    const int nt = std::max(1, std::min<int>(8, int(std::thread::hardware_concurrency())));
+    auto parallel_rows = [&](auto&& body) {
+        if (nt > 1 && n > 16384) {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t]() { body(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt)); });
+            for (auto& x : th) x.join();
+        } else {
+            body(0, n);
         }
-        (void)w;
-    }
-    for (int64_t s = 0; s < ns; ++s) cptr[s + 1] += cptr[s];
+    };
+    parallel_rows([&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            for (int k = ig[i]; k < ig[i + 1]; ++k) {
+                int64_t c = 0;
+                common(i, jg[k], [&](int) { ++c; });
+                cptr[k + 1] = c;
+            }
+            cptr[nnz + i + 1] = eptr[i + 1] - eptr[i];
+        }
+    });
+    for (int64_t k = 0; k < ns; ++k) cptr[k + 1] += cptr[k];
+    if (cptr[ns] != int64_t(kpar) * 78) throw Error(0x0004, "assembly: pattern lacks an element entry");
     codes.resize(cptr[ns]);
-    std::vector<int64_t> fill(cptr.begin(), cptr.end() - 1);
-    for (int e = 0; e < kpar; ++e) {
-        const int* E = ed + 12 * e;
-        for (int a = 0; a < 12; ++a) {
-            const int ii = E[a];
-            for (int b = 0; b < 12; ++b) {
-                const int jj = E[b];
-                int64_t s = -1;
-                if (jj < ii) {
-                    const int* lo = jg + ig[ii];
-                    const int* it = std::lower_bound(lo, jg + ig[ii + 1], jj);
-                    s = it - jg;
-                } else if (jj == ii && a == b) {
-                    s = nnz + ii;
-                }
-                if (s >= 0) codes[fill[s]++] = uint32_t(e) * 144u + uint32_t(a * 12 + b);
+    parallel_rows([&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            for (int k = ig[i]; k < ig[i + 1]; ++k) {
+                int64_t q = cptr[k];
+                const int j = jg[k];
+                common(i, j, [&](int e) { codes[q++] = uint32_t(e) * 144u + uint32_t(local(e, i) * 12 + local(e, j)); });
+            }
+            int64_t q = cptr[nnz + i];
+            for (int p = eptr[i]; p < eptr[i + 1]; ++p) {
+                const int e = eel[p], a = local(e, i);
+                codes[q++] = uint32_t(e) * 144u + uint32_t(a * 13);
             }
         }
-    }
-    (void)slot;
+    });
 }
 
 Assembler::Assembler(const emtd_mesh& m, int n, const int* ig, const int* jg, hipStream_t s) : stream(s) {

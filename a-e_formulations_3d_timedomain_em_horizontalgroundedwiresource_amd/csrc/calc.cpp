@@ -55,6 +55,18 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     const double t0 = wall_seconds();
     select_device(device);
     M_ = read_model_files(dir, form);
+    // phase times of the setup, printed with EMTD_SETUP_TIMES=1
+    const bool show_times = std::getenv("EMTD_SETUP_TIMES") != nullptr;
+    double tmark = t0;
+    std::string times;
+    auto mark = [&](const char* what) {
+        const double t = wall_seconds();
+        char b[96];
+        std::snprintf(b, sizeof b, " %s %.3f", what, t - tmark);
+        times += b;
+        tmark = t;
+    };
+    mark("read");
     form_ = form >= 0 ? form : (M_.variant_form >= 0 ? M_.variant_form : 0);
     scheme_ = scheme >= 0 ? scheme : (M_.variant_scheme >= 0 ? M_.variant_scheme : 2);
     if (scheme_ != 2 && scheme_ != 3) throw Error(EMTD_ERR_RANGE, "scheme must be 2 or 3");
@@ -73,10 +85,12 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     std::vector<GridCoord> coords = edge_grid_coords(M_.kuzlov, M_.xyz.data(), n_, M_.edges.data());
     OrderingOptions opt;
     S_ = std::make_unique<Symbolic>(analyze(n_, ig_.data(), jg_.data(), active, coords, opt));
+    mark("pattern+ordering+symbolic");
     nn_ = S_->nn;
     // device memory is claimed only after the caller admitted the footprint
     footprint_ = estimate_footprint();
     if (admit) admit(footprint_);
+    mark("admission");
     EMTD_HIP(hipStreamCreateWithFlags(&sown_.s, hipStreamNonBlocking));
     stream_ = sown_.s;
     keep_pool_memory(device);
@@ -98,12 +112,14 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         assembly_gpu_seconds_ = A.last_seconds();
         assembly_bytes_ = double(A.bytes());
     }
+    mark("assembly");
     {
         std::vector<int64_t> doff, ddiag;
         assembly_map(*S_, n_, ig_.data(), jg_.data(), doff, ddiag);
         dest_off_ = to_device(doff, stream_);
         dest_diag_ = to_device(ddiag, stream_);
     }
+    mark("assembly_map");
     if (precision_ == EMTD_FP64) {
         c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
         if (prefetch_) {
@@ -117,6 +133,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     } else
         c32_ = std::make_unique<GpuCholesky<float>>(*S_, stream_);
     perm_ = to_device(S_->perm, stream_);
+    mark("solver");
     if (precision_ == EMTD_FP32_IR) build_residual_operator();
     // primary/secondary mode (A2/time_approx.cpp:350-370, 456-468): the (sigma - sigma0) mass
     // matrix C_dsig, Assembling(4) of the reference, on the same pattern
@@ -145,35 +162,57 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         EMTD_HIP(hipStreamSynchronize(stream_));
         c_host_ggl_ = cg;
         c_host_di_ = cd;
-        struct Ent {
-            int col;
-            double c, d;
-            bool operator<(const Ent& o) const { return col < o.col; }
-        };
-        std::vector<std::vector<Ent>> rows(n_);
+        // counting build: entries per permuted row, then each (short) row sorted by column
+        std::vector<int64_t> rp(n_ + 1, 0);
+        auto keep = [&](double c, double d) { return c != 0.0 || d != 0.0; };
         for (int i = 0; i < n_; ++i) {
             const int a = S_->iperm[i];
-            const double di = ps_ ? dd[i] : 0.0;
-            if (cd[i] != 0.0 || di != 0.0) rows[a].push_back({a, cd[i], di});
+            if (keep(cd[i], ps_ ? dd[i] : 0.0)) ++rp[a + 1];
             for (int k = ig_[i]; k < ig_[i + 1]; ++k) {
-                const double dk = ps_ ? dg[k] : 0.0;
-                if (cg[k] == 0.0 && dk == 0.0) continue;
-                const int b = S_->iperm[jg_[k]];
-                rows[a].push_back({b, cg[k], dk});
-                rows[b].push_back({a, cg[k], dk});
+                if (!keep(cg[k], ps_ ? dg[k] : 0.0)) continue;
+                ++rp[a + 1];
+                ++rp[S_->iperm[jg_[k]] + 1];
             }
         }
-        std::vector<int64_t> rp(n_ + 1, 0);
-        std::vector<int> ci;
-        std::vector<double> cv, cdv;
-        for (int a = 0; a < n_; ++a) {
-            std::sort(rows[a].begin(), rows[a].end());
-            for (auto& e : rows[a]) {
-                ci.push_back(e.col);
-                cv.push_back(e.c);
-                if (ps_) cdv.push_back(e.d);
+        for (int a = 0; a < n_; ++a) rp[a + 1] += rp[a];
+        std::vector<int> ci(rp[n_]);
+        std::vector<double> cv(rp[n_]), cdv(ps_ ? rp[n_] : 0);
+        {
+            std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+            auto put = [&](int row, int col, double c, double d) {
+                const int64_t q = pos[row]++;
+                ci[q] = col;
+                cv[q] = c;
+                if (ps_) cdv[q] = d;
+            };
+            for (int i = 0; i < n_; ++i) {
+                const int a = S_->iperm[i];
+                const double di = ps_ ? dd[i] : 0.0;
+                if (keep(cd[i], di)) put(a, a, cd[i], di);
+                for (int k = ig_[i]; k < ig_[i + 1]; ++k) {
+                    const double dk = ps_ ? dg[k] : 0.0;
+                    if (!keep(cg[k], dk)) continue;
+                    const int b = S_->iperm[jg_[k]];
+                    put(a, b, cg[k], dk);
+                    put(b, a, cg[k], dk);
+                }
             }
-            rp[a + 1] = int64_t(ci.size());
+        }
+        for (int a = 0; a < n_; ++a) {  // insertion sort of each row by column
+            for (int64_t q = rp[a] + 1; q < rp[a + 1]; ++q) {
+                const int c = ci[q];
+                const double v = cv[q], w = ps_ ? cdv[q] : 0.0;
+                int64_t r = q;
+                while (r > rp[a] && ci[r - 1] > c) {
+                    ci[r] = ci[r - 1];
+                    cv[r] = cv[r - 1];
+                    if (ps_) cdv[r] = cdv[r - 1];
+                    --r;
+                }
+                ci[r] = c;
+                cv[r] = v;
+                if (ps_) cdv[r] = w;
+            }
         }
         crp_ = to_device(rp, stream_);
         cci_ = to_device(ci, stream_);
@@ -181,6 +220,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         if (ps_) cdv_ = to_device(cdv, stream_);
         c_nnz_ = int64_t(ci.size());
     }
+    mark("mass_csr");
     if (ps_) {
         for (int i = 0; i < n_; ++i) {
             if (!M_.anomal[i]) continue;
@@ -230,6 +270,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
             }
         u0_ = to_device(u0, stream_);
     }
+    mark("initial_field");
     // line sources
     {
         SourceInfo si = place_line_sources(M_);
@@ -264,6 +305,7 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
         rec_idx_ = to_device(idx.empty() ? std::vector<int>{0} : idx, stream_);
         rec_w_ = to_device(w.empty() ? std::vector<double>{0.0} : w, stream_);
     }
+    mark("sources+receivers");
     print_window(M_, first_, last_);
     plan_ = decade_plan(M_.times);
     const int nt_out = last_ - first_ + 1;
@@ -271,7 +313,9 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     EMTD_HIP(hipMemsetAsync(rec_vals_.p, 0, rec_vals_.n * sizeof(double), stream_));
     rec_done_.assign(M_.times.size(), 0);
     EMTD_HIP(hipStreamSynchronize(stream_));
+    mark("rest");
     setup_seconds_ = wall_seconds() - t0;
+    if (show_times) std::fprintf(stderr, "[emtd setup] n=%d%s\n", n_, times.c_str());
 }
 
 CalcCore::~CalcCore() {

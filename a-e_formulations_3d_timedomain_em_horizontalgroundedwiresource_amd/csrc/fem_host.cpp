@@ -2,6 +2,7 @@
 #include "fem_host.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -11,27 +12,55 @@
 namespace emtd {
 
 void build_portrait(int kpar, const int* ed, int n, std::vector<int>& ig, std::vector<int>& jg) {
-    std::vector<int> cnt(n + 1, 0);
-    // rows gather candidates; duplicates removed per row
-    std::vector<std::vector<int>> rows(n);
-    for (int e = 0; e < kpar; ++e) {
-        const int* E = ed + 12 * e;
-        for (int a = 0; a < 12; ++a)
+    // row i holds the smaller edges of every element that has edge i (sorted, unique); rows are
+    // built independently from the edge -> element lists, in parallel, counted then filled
+    std::vector<int> eptr(n + 1, 0), eel(size_t(kpar) * 12);
+    for (int64_t q = 0; q < int64_t(kpar) * 12; ++q) eptr[ed[q] + 1]++;
+    for (int i = 0; i < n; ++i) eptr[i + 1] += eptr[i];
+    {
+        std::vector<int> fill(eptr.begin(), eptr.end() - 1);
+        for (int e = 0; e < kpar; ++e)
+            for (int a = 0; a < 12; ++a) eel[fill[ed[12 * e + a]]++] = e;
+    }
+    auto row = [&](int i, int* buf) {  // returns the count; buf holds >= 12 * (elements of i)
+        int c = 0;
+        for (int p = eptr[i]; p < eptr[i + 1]; ++p) {
+            const int* E = ed + 12 * eel[p];
             for (int b = 0; b < 12; ++b)
-                if (E[b] < E[a]) rows[E[a]].push_back(E[b]);
-    }
-    ig.assign(n + 1, 0);
-    int64_t total = 0;
-    for (int i = 0; i < n; ++i) {
-        std::vector<int>& r = rows[i];
-        std::sort(r.begin(), r.end());
-        r.erase(std::unique(r.begin(), r.end()), r.end());
-        total += int64_t(r.size());
-        if (total > INT32_MAX) throw Error(0x0004, "pattern exceeds 32-bit RSF indices");
-        ig[i + 1] = int(total);
-    }
-    jg.resize(total);
-    for (int i = 0; i < n; ++i) std::copy(rows[i].begin(), rows[i].end(), jg.begin() + ig[i]);
+                if (E[b] < i) buf[c++] = E[b];
+        }
+        std::sort(buf, buf + c);
+        return int(std::unique(buf, buf + c) - buf);
+    };
+    int maxe = 0;
+    for (int i = 0; i < n; ++i) maxe = std::max(maxe, eptr[i + 1] - eptr[i]);
+    const int nt = std::max(1, std::min<int>(8, int(std::thread::hardware_concurrency())));
+    auto parallel_rows = [&](auto&& body) {
+        if (nt > 1 && n > 16384) {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t]() { body(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt)); });
+            for (auto& x : th) x.join();
+        } else {
+            body(0, n);
+        }
+    };
+    std::vector<int64_t> cnt(n + 1, 0);
+    parallel_rows([&](int i0, int i1) {
+        std::vector<int> buf(size_t(12) * std::max(maxe, 1));
+        for (int i = i0; i < i1; ++i) cnt[i + 1] = row(i, buf.data());
+    });
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    if (cnt[n] > INT32_MAX) throw Error(0x0004, "pattern exceeds 32-bit RSF indices");
+    ig.assign(cnt.begin(), cnt.end());
+    jg.resize(cnt[n]);
+    parallel_rows([&](int i0, int i1) {
+        std::vector<int> buf(size_t(12) * std::max(maxe, 1));
+        for (int i = i0; i < i1; ++i) {
+            const int c = row(i, buf.data());
+            std::copy(buf.begin(), buf.begin() + c, jg.begin() + ig[i]);
+        }
+    });
 }
 
 std::vector<char> boundary_edge_mask(const ModelFiles& M) {

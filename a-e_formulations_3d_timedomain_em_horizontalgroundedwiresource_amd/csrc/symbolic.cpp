@@ -15,6 +15,8 @@
 #include "symbolic.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <map>
 #include <climits>
 #include <cmath>
@@ -441,34 +443,50 @@ void assembly_map(const Symbolic& S, int n, const int* ig, const int* jg,
     std::vector<int> front_of(S.nn);
     for (int f = 0; f < S.nf; ++f)
         for (int k = S.fbeg[f]; k < S.fbeg[f + 1]; ++k) front_of[k] = f;
+    std::atomic<bool> bad{false};
     auto pos_in_front = [&](int f, int r) -> int {
         const int fb = S.fbeg[f], fe = S.fbeg[f + 1];
         if (r < fe) return r - fb;
         const int* b = S.rows.data() + S.rows_ptr[f] + (fe - fb);
         const int* e = S.rows.data() + S.rows_ptr[f + 1];
         const int* it = std::lower_bound(b, e, r);
-        if (it == e || *it != r) throw Error(0x0080, "assembly map: row outside front structure");
+        if (it == e || *it != r) {
+            bad.store(true, std::memory_order_relaxed);
+            return 0;
+        }
         return (fe - fb) + int(it - b);
     };
     dest_off.assign(ig[n], -1);
     dest_diag.assign(n, -1);
-    for (int i = 0; i < n; ++i) {
-        int a = S.iperm[i];
-        if (a < S.nn) {
-            int f = front_of[a];
-            int lc = a - S.fbeg[f];
-            dest_diag[i] = S.loff[f] + int64_t(lc) * S.mcount[f] + lc;
+    // rows are independent: each writes its own entries
+    auto work = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            int a = S.iperm[i];
+            if (a < S.nn) {
+                int f = front_of[a];
+                int lc = a - S.fbeg[f];
+                dest_diag[i] = S.loff[f] + int64_t(lc) * S.mcount[f] + lc;
+            }
+            for (int k = ig[i]; k < ig[i + 1]; ++k) {
+                int b = S.iperm[jg[k]];
+                if (a >= S.nn || b >= S.nn) continue;
+                int col = std::min(a, b), row = std::max(a, b);
+                int f = front_of[col];
+                int lc = col - S.fbeg[f];
+                int lr = pos_in_front(f, row);
+                dest_off[k] = S.loff[f] + int64_t(lc) * S.mcount[f] + lr;
+            }
         }
-        for (int k = ig[i]; k < ig[i + 1]; ++k) {
-            int b = S.iperm[jg[k]];
-            if (a >= S.nn || b >= S.nn) continue;
-            int col = std::min(a, b), row = std::max(a, b);
-            int f = front_of[col];
-            int lc = col - S.fbeg[f];
-            int lr = pos_in_front(f, row);
-            dest_off[k] = S.loff[f] + int64_t(lc) * S.mcount[f] + lr;
-        }
+    };
+    const int nt = std::max(1, std::min<int>(8, int(std::thread::hardware_concurrency())));
+    if (nt > 1 && n > 16384) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt));
+        for (auto& x : th) x.join();
+    } else {
+        work(0, n);
     }
+    if (bad) throw Error(0x0080, "assembly map: row outside front structure");
 }
 
 }  // namespace emtd
