@@ -2,7 +2,7 @@
 
 Usage: ``python -m <package>.build`` or ``build()`` from Python. Compiles every source under
 ``csrc/`` with hipcc (objects under ``_build/``), links ``libemtd.so`` next to this file, and
-builds the drop-in executable ``calcnonstat3dline``.
+builds the drop-in executables ``calcnonstat3dline``, ``sumnonstat3donly`` and ``summatoredmn``.
 """
 from __future__ import annotations
 
@@ -16,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemtd.so")
 EXE = os.path.join(HERE, "calcnonstat3dline")
+# drop-in executables: CalcNonstat3DLine, SumNonstat3DOnly, SummatorEDMN
+MAINS = {"calc_main.cpp": "calcnonstat3dline", "sum_main.cpp": "sumnonstat3donly", "edmn_main.cpp": "summatoredmn"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -35,7 +37,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
-                  if f.endswith((".hip", ".cpp")) and f != "calc_main.cpp")
+                  if f.endswith((".hip", ".cpp")) and f not in MAINS)
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
@@ -45,14 +47,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    main_src = os.path.join(CSRC, "calc_main.cpp")
-    if os.path.exists(main_src):
-        cmd = [HIPCC, "-O2", "-std=c++17", main_src, "-o", EXE + ".tmp", f"-L{HERE}", "-lemtd",
+    for main, exe in MAINS.items():
+        main_src = os.path.join(CSRC, main)
+        out = os.path.join(HERE, exe)
+        cmd = [HIPCC, "-O2", "-std=c++17", main_src, "-o", out + ".tmp", f"-L{HERE}", "-lemtd",
                "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"executable link failed:\n{r.stdout}\n{r.stderr}")
-        os.replace(EXE + ".tmp", EXE)
+        os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
     return LIB

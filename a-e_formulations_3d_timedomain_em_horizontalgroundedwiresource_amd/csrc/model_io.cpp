@@ -134,27 +134,7 @@ ModelFiles read_model_files(const std::string& dir, int formulation) {
         M.mu.resize(M.nmat);
         for (int i = 0; i < M.nmat; ++i) M.mu[i] = mur[i] * kMu0;
     }
-    {  // infite.0 (A2/vec_prep_data.cpp:834-853)
-        std::string txt = read_text(P("infite.0"));
-        size_t pos = txt.find('=');
-        if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
-        int ntime = std::atoi(txt.c_str() + pos + 1);
-        for (int k = 0; k < 4; ++k) {
-            pos = txt.find('\n', pos);
-            if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
-            ++pos;
-        }
-        const char* c = txt.c_str() + pos;
-        M.times.resize(ntime);
-        for (int i = 0; i < ntime; ++i) {
-            char* end;
-            M.times[i] = std::strtod(c, &end);
-            if (end == c) throw Error(0x0040, "bad time value in infite.0");
-            c = std::strchr(end, ';');
-            if (!c) { if (i + 1 < ntime) throw Error(0x0040, "truncated infite.0"); break; }
-            ++c;
-        }
-    }
+    M.times = read_time_grid(P("infite.0"));
     M.recB = read_points(P("xyzVectorB"));
     M.recE = read_points(P("xyzVectorE"));
     {  // group, recvsb, recvse (A2/TaskSP.cpp:467-535)
@@ -230,6 +210,31 @@ ModelFiles read_model_files(const std::string& dir, int formulation) {
     else
         M.v3a = read_bin<double>(P("v3a.dat"), size_t(M.npls) * M.kuzlov);
     return M;
+}
+
+std::vector<double> read_time_grid(const std::string& path) {
+    // infite.0 (A2/vec_prep_data.cpp:834-853): the count after the first '=', four lines
+    // further the ';'-separated times
+    std::string txt = read_text(path);
+    size_t pos = txt.find('=');
+    if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
+    int ntime = std::atoi(txt.c_str() + pos + 1);
+    for (int k = 0; k < 4; ++k) {
+        pos = txt.find('\n', pos);
+        if (pos == std::string::npos) throw Error(0x0040, "bad infite.0");
+        ++pos;
+    }
+    const char* c = txt.c_str() + pos;
+    std::vector<double> times(ntime);
+    for (int i = 0; i < ntime; ++i) {
+        char* end;
+        times[i] = std::strtod(c, &end);
+        if (end == c) throw Error(0x0040, "bad time value in infite.0");
+        c = std::strchr(end, ';');
+        if (!c) { if (i + 1 < ntime) throw Error(0x0040, "truncated infite.0"); break; }
+        ++c;
+    }
+    return times;
 }
 
 void load_normal_field(const ModelFiles& M, int tnum, float* out) {

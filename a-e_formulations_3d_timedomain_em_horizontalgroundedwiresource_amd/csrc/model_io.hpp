@@ -44,6 +44,9 @@ void load_normal_field(const ModelFiles& M, int tnum, float* out);
 // 982-1045; A2/time_approx.cpp:243-287; A2/TaskSP.cpp:467-535; A2/give_out_vec_loop.cpp:46-73).
 ModelFiles read_model_files(const std::string& dir, int formulation);
 
+// The time grid of infite.0 (A2/vec_prep_data.cpp:834-853).
+std::vector<double> read_time_grid(const std::string& path);
+
 std::string join_path(const std::string& dir, const std::string& name);
 bool file_exists(const std::string& path);
 

@@ -305,11 +305,13 @@ def write_model(spec: ModelSpec, out: str) -> dict:
     with open(os.path.join(out, "group"), "w") as f:
         f.write(f"1\n1 1 {npls}\n")
     rec = spec.receivers if spec.receivers is not None else default_receivers()
+    # the receiver files list each source's receivers in turn (recvsb/recvse hold the counts)
     for name in ("xyzVectorB", "xyzVectorE"):
         with open(os.path.join(out, name), "w") as f:
-            f.write(f"{len(rec)}\n")
-            for r in rec:
-                f.write(f"{r[0]:.17g} {r[1]:.17g} {r[2]:.17g}\n")
+            f.write(f"{len(rec) * npls}\n")
+            for _ in range(npls):
+                for r in rec:
+                    f.write(f"{r[0]:.17g} {r[1]:.17g} {r[2]:.17g}\n")
     for name in ("recvsb", "recvse"):
         with open(os.path.join(out, name), "w") as f:
             f.write("".join(f"{len(rec)}\n" for _ in range(npls)))

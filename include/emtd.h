@@ -164,6 +164,19 @@ void emtd_calc_destroy(emtd_calc* c);
 int emtd_calc_nonstat_3d_line(const char* workdir, int formulation, int scheme, int precision,
                               int device);
 
+/* ---------------------------------------------------------------- output chain (host only)
+ * Replace SumNonstat3DOnly (M/SumNonstat3DOnly/main.cpp:171-384): the module's tables
+ * ball_anom.N, edsall_anom.N, eall_anom.N in workdir become one file per component (Bxall.N,
+ * Byall.N, Bzall.N, Edsxall.N, Edsyall.N, Edsall.N, Exall.N, Eyall.N, Ezall.N) with rows
+ * (t ms, normal = 0, anomalous, total). */
+int emtd_sum_nonstat_3d_only(const char* workdir);
+/* Replaces SummatorEDMN (M/SummatorEDMN/main.cpp:40-295, called as "SummatorEDMN WithDDV profile
+ * place"): integrates exall/eyall/ezall<profile>_<place> along the MN lines of
+ * xyzmn<place+1>_<profile> (points in xyzVectorElin<place+1>_<profile>) into
+ * edmnall<profile>_<place>, and with with_ddv != 0 also edmoonall<profile>_<place>. Does nothing
+ * when the file xyzmn lists no line. */
+int emtd_summator_edmn(const char* workdir, int with_ddv, int profile, int place);
+
 /* ---------------------------------------------------------------- request loop
  * No counterpart in the reference (which runs one model per process). For callers that serve
  * many forward-model requests on one GPU: each request is a calculation directory with the

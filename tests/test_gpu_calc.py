@@ -5,6 +5,7 @@ the oracle on the same files, and compares the receiver values (undifferentiated
 three output tables. FP64 bar: 1e-6 relative (per table, against its max magnitude).
 """
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -333,3 +334,56 @@ def test_calc_lookahead_factorization_is_exact(emtd, tmp_path, monkeypatch, form
     ref = calc.run(m, form, scheme)
     for got, want in zip(vals["1"], (ref.valB, ref.valE)):
         assert np.abs(got - want).max() / np.abs(want).max() < TOL64
+
+
+def test_output_chain_executables_match_oracle(emtd, mg, tmp_path):
+    """Module -> SumNonstat3DOnly -> (the orchestrator's rename exall.N -> exall<profile>_<place>)
+    -> SummatorEDMN through the three drop-in executables, against the same chain on the
+    oracle's tables: the MN-line voltages agree at the FP64 bar."""
+    import subprocess
+    from oracle import summators
+    d, spec = make_model(tmp_path / "gpu", cfg=1, cells=(8, 8, 6), steps=(4, 2))
+    rec = spec.receivers
+    dx = rec[1, 0] - rec[0, 0]
+    a = (rec[0, 0] - 0.5 * dx, rec[0, 1], 0.0)
+    b = (rec[-1, 0] + 0.5 * dx, rec[0, 1], 0.0)
+    prof, place = 2, 0
+
+    def add_lines(dd):
+        with open(os.path.join(dd, "xyzmn"), "w") as f:
+            f.write("1\n")
+        with open(os.path.join(dd, f"xyzmn{place + 1}_{prof}"), "w") as f:
+            f.write("1\n" + " ".join(f"{v:.16e}" for v in (*a, *b)) + "\n")
+        with open(os.path.join(dd, f"xyzVectorElin{place + 1}_{prof}"), "w") as f:
+            f.write(f"{len(rec)}\n" + "".join(f"{r[0]:.16e} {r[1]:.16e} {r[2]:.16e}\n" for r in rec))
+
+    def rename(dd):
+        for c in "xyz":
+            os.replace(os.path.join(dd, f"E{c}all.1"), os.path.join(dd, f"e{c}all{prof}_{place}"))
+
+    od = str(tmp_path / "oracle")
+    shutil.copytree(d, od)
+    exe = lambda name: os.path.join(os.path.dirname(emtd.LIB_PATH), name)  # noqa: E731
+    for cmd in ([exe("calcnonstat3dline"), "-d", d], [exe("sumnonstat3donly"), "-d", d]):
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    rename(d)
+    add_lines(d)
+    r = subprocess.run([exe("summatoredmn"), "1", str(prof), str(place), "-d", d], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    m = refio.read_model(od)
+    ref = calc.run(m, "A", 2)
+    refio.write_tables(od, m, ref.first, ref.last, ref.eds, ref.bfield, ref.efield, "A")
+    summators.sum_nonstat_3d_only(od)
+    rename(od)
+    add_lines(od)
+    summators.summator_edmn(od, True, prof, place)
+    for name in (f"edmnall{prof}_{place}", f"edmoonall{prof}_{place}"):
+        got, want = (summators._read_picket_file(os.path.join(x, name), 1)[0] for x in (d, od))
+        np.testing.assert_allclose(got[0], want[0], rtol=1e-15)
+        scale = np.abs(want[1]).max()
+        assert scale > 0
+        err = np.abs(got[1] - want[1]).max() / scale
+        print(name, "max rel err", err)
+        assert err < TOL64

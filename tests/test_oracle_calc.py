@@ -244,8 +244,9 @@ def test_sources_superpose(tmp_path):
         runs[name] = calc.run(m, "A", 2)
     for k, single in ((0, "w0"), (1, "w1")):
         for attr in ("valB", "valE"):
-            a = getattr(runs["both"], attr)[:, k]
             b = getattr(runs[single], attr)[:, 0]
+            # the two-source model lists each source's receivers in turn: source k's own set
+            a = getattr(runs["both"], attr)[:, k, k * b.shape[1]:(k + 1) * b.shape[1]]
             scale = np.abs(b).max()
             assert scale > 0
             assert np.abs(a - b).max() / scale < 1e-10, (k, attr)
