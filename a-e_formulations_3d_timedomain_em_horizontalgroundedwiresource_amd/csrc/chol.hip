@@ -861,14 +861,67 @@ __global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, 
     for (int i = 0; i < SB; ++i) out[i * SB + t] = T(X[i][t]);
 }
 
+// The links' shortcut blocks, per pivot block b of a split-sweep front (after k_invdiag):
+//   M_b = inv(L_bb) L_{b,b-1}        (b >= 1; forward:  y_b = inv(L_bb)(v_b - ...) - M_b y_{b-1})
+//   K_b = inv(L_bb)^T L_{b+1,b}^T    (b + 1 < np; backward: x_b = inv(L_bb)^T(...) - K_b x_{b+1})
+// both stored column-major ([col][row], so a wavefront reads a column with consecutive lanes),
+// zero outside the valid rows/columns.
+template <typename T>
+__global__ void __launch_bounds__(256) k_chainblk(const DevFront* __restrict__ F, const int2* __restrict__ items,
+                                                  const int64_t* __restrict__ ioff, const T* __restrict__ L,
+                                                  const T* __restrict__ inv, T* __restrict__ mblk, T* __restrict__ kblk) {
+    __shared__ double A[SB][SB + 1];  // inv(L_bb), [row][col]
+    __shared__ double Bm[SB][SB + 1];
+    const int2 it = items[blockIdx.x];
+    const DevFront Fr = F[it.x];
+    const int b = it.y, p = Fr.p, np = (p + 63) / 64;
+    const int nb = min(64, p - 64 * b);
+    const int64_t m = Fr.m;
+    const T* Lf = L + Fr.loff;
+    const T* iv = inv + ioff[it.x] + int64_t(b) * SB * SB;
+    const int t = threadIdx.x;
+    for (int e = t; e < SB * SB; e += 256) A[e / SB][e % SB] = double(iv[e]);
+    const int64_t base = ioff[it.x] + int64_t(b) * SB * SB;
+    // M_b[i][j] = sum_k inv[i][k] L[64b + k][64(b-1) + j]
+    __syncthreads();
+    for (int e = t; e < SB * SB; e += 256) {
+        const int k = e % SB, j = e / SB;  // coalesced along k (rows of L)
+        Bm[k][j] = (b >= 1 && k < nb) ? double(Lf[(64 * (b - 1) + j) * m + 64 * b + k]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < SB * SB; e += 256) {
+        const int i = e % SB, j = e / SB;
+        double acc = 0.0;
+        if (b >= 1)
+            for (int k = 0; k < SB; ++k) acc += A[i][k] * Bm[k][j];
+        mblk[base + int64_t(j) * SB + i] = T(acc);
+    }
+    __syncthreads();
+    // K_b[i][j] = sum_k inv[k][i] L[64(b+1) + j][64b + k]
+    const int nb1 = b + 1 < np ? min(64, p - 64 * (b + 1)) : 0;
+    for (int e = t; e < SB * SB; e += 256) {
+        const int j = e % SB, k = e / SB;  // coalesced along j (rows of L)
+        Bm[k][j] = (j < nb1 && k < nb) ? double(Lf[(64 * b + k) * m + 64 * (b + 1) + j]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < SB * SB; e += 256) {
+        const int i = e % SB, j = e / SB;
+        double acc = 0.0;
+        if (nb1 > 0)
+            for (int k = 0; k < SB; ++k) acc += A[k][i] * Bm[k][j];
+        kblk[base + int64_t(j) * SB + i] = T(acc);
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ ybuf, int* __restrict__ counter,
                                                    SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
-                                                   const T* __restrict__ inv, const int64_t* __restrict__ ioff,
-                                                   double* __restrict__ W, double* __restrict__ x) {
+                                                   const T* __restrict__ inv, const T* __restrict__ mblk,
+                                                   const int64_t* __restrict__ ioff, double* __restrict__ W,
+                                                   double* __restrict__ x) {
     __shared__ int s_ticket;
-    __shared__ double red[4][64];
+    __shared__ double red[4][64], red2[4][64], red3[4][64];
     __shared__ double rs[64];
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
@@ -882,14 +935,17 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     double* yb = ybuf + int64_t(foff[it.x]) * 64;
-    // loaded before any wait: this thread's 16 entries of inv(L_bb) (row lr, columns
-    // 16q..16q+15) and the gathered right-hand side of its row
-    const T* iv = inv + ioff[it.x] + int64_t(rb) * SB * SB + lr * SB + 16 * q;
-    double ivr[16];
+    // loaded before any wait: this thread's 16 entries of inv(L_bb) and of M_b = inv(L_bb) L_{b,b-1}
+    // (row lr, columns 16q..16q+15) and the gathered right-hand side of its row
+    const int64_t boff = ioff[it.x] + int64_t(rb) * SB * SB;
+    const T* iv = inv + boff + lr * SB + 16 * q;
+    double ivr[16], mr[16];
     for (int j = 0; j < 16; ++j) ivr[j] = double(iv[j]);
+    for (int j = 0; j < 16; ++j) mr[j] = rb > 0 ? double(mblk[boff + int64_t(16 * q + j) * SB + lr]) : 0.0;
     const double vr = (q == 0 && lr < nbp) ? v[r] : 0.0;
+    // t = v_b - sum_{c < b-1} L_bc y_c as the earlier blocks arrive
     double acc = 0.0;
-    for (int cb = 0; cb < rb; ++cb) {
+    for (int cb = 0; cb + 1 < rb; ++cb) {
         double lv[16], yv[16];
         const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
         for (int j = 0; j < 16; ++j) lv[j] = lr < nbp ? double(lc[int64_t(j) * m]) : 0.0;
@@ -900,14 +956,24 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     __syncthreads();
     if (q == 0) rs[lr] = lr < nbp ? vr - (red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr]) : 0.0;
     __syncthreads();
-    // y = inv(L_bb) rs : thread (row lr, quarter q) sums 16 columns
+    // w = inv(L_bb) t : thread (row lr, quarter q) sums 16 columns
     double part = 0.0;
     for (int j = 0; j < 16; ++j) part += ivr[j] * rs[16 * q + j];
-    red[q][lr] = part;
+    red2[q][lr] = part;
+    // y_b = w - M_b y_{b-1}: after the previous block's result only this product remains
+    double pm = 0.0;
+    if (rb > 0) {
+        double yv[16];
+        poll16(yb + 64 * (rb - 1) + 16 * q, yv, ws, rb - 1, 16 * q);
+        for (int j = 0; j < 16; ++j) pm += mr[j] * yv[j];
+    }
+    red3[q][lr] = pm;
     __syncthreads();
     if (q == 0) {
         // rows past the block end publish 0 so that readers of whole 16-slot groups finish
-        const double y = lr < nbp ? red[0][lr] + red[1][lr] + red[2][lr] + red[3][lr] : 0.0;
+        const double y = lr < nbp ? (red2[0][lr] + red2[1][lr] + red2[2][lr] + red2[3][lr]) -
+                                        (red3[0][lr] + red3[1][lr] + red3[2][lr] + red3[3][lr])
+                                  : 0.0;
         if (lr < nbp) {
             v[r] = y;
             x[Fr.fbeg + r] = y;
@@ -1008,20 +1074,21 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
 // Backward chain link for pivot column block cb of a front. Everything that does not depend on
 // the next block's result is done before waiting for it: the partial sums of L21^T x_U and the
 // gathered y_b, the blocks cb2 > cb+1 (coalesced reads, lane = row, reduced over lanes once),
-// and the loads of the 64x64 block L[cb+1 rows][cb columns] and of this block's inverse; after
-// the flag of block cb+1 only 16 products per thread, two LDS reductions and the 64x64 GEMV
-// with the inverse remain on the critical path.
+// and z = inv(L_bb)^T (y_b - t_b - sum_{cb2 > cb+1} L_{cb2,cb}^T x_cb2); after the next block's
+// result only x_b = z - K_b x_{cb+1} remains (K_b = inv(L_bb)^T L_{cb+1,cb}^T, 16 products per
+// thread and one LDS reduction).
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ xbuf, int* __restrict__ counter,
                                                    SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
-                                                   const T* __restrict__ inv, const int64_t* __restrict__ ioff,
+                                                   const T* __restrict__ inv, const T* __restrict__ kblk,
+                                                   const int64_t* __restrict__ ioff,
                                                    const double* __restrict__ part, const int64_t* __restrict__ poff,
                                                    double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
     __shared__ double sred[64];
     __shared__ double rs[64];
-    __shared__ double red[4][64];
+    __shared__ double red2[4][64], red3[4][64];
     if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
     __syncthreads();
     const int2 it = items[s_ticket];
@@ -1034,18 +1101,14 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
-    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15)
-    const T* iv = inv + ioff[it.x] + int64_t(cb) * SB * SB + lane;
-    double ivc[16];
-    for (int j = 0; j < 16; ++j) ivc[j] = double(iv[(16 * wave + j) * SB]);
-    // L[64(cb+1) + 16w + j][c0 + lane]: the next block's rows against this thread's column
+    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15) and of K_b (row lane,
+    // columns 16w..16w+15)
+    const int64_t boff = ioff[it.x] + int64_t(cb) * SB * SB;
+    const T* iv = inv + boff + lane;
     const bool has_next = cb + 1 < np;
-    const int nb1 = has_next ? min(64, p - 64 * (cb + 1)) : 0;
-    double ln[16];
-    for (int j = 0; j < 16; ++j) {
-        const int rr = 16 * wave + j;
-        ln[j] = (lane < nbc && rr < nb1) ? double(Lf[int64_t(c0 + lane) * m + 64 * (cb + 1) + rr]) : 0.0;
-    }
+    double ivc[16], kr[16];
+    for (int j = 0; j < 16; ++j) ivc[j] = double(iv[(16 * wave + j) * SB]);
+    for (int j = 0; j < 16; ++j) kr[j] = has_next ? double(kblk[boff + int64_t(16 * wave + j) * SB + lane]) : 0.0;
     // y_b - L21^T x_U for this thread's column (wave 0)
     double base = 0.0;
     if (wave == 0 && lane < nbc) {
@@ -1072,25 +1135,26 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
         for (int o = 32; o > 0; o >>= 1) sv += __shfl_down(sv, o, 64);
         if (lane == 0) sred[wave + 4 * j] = sv;
     }
-    // block cb+1 (its rows past the block end are published as 0)
-    double pn = 0.0;
+    __syncthreads();
+    if (wave == 0) rs[lane] = lane < nbc ? base - sred[lane] : 0.0;
+    __syncthreads();
+    // z = inv(L_bb)^T rs : thread (column lane, quarter) sums 16 rows of inv
+    double pt = 0.0;
+    for (int j = 0; j < 16; ++j) pt += ivc[j] * rs[16 * wave + j];
+    red2[wave][lane] = pt;
+    // x_b = z - K_b x_{cb+1} (its rows past the block end are published as 0)
+    double pk = 0.0;
     if (has_next) {
         double xn[16];
         poll16(xb + 64 * (cb + 1) + 16 * wave, xn, ws3, cb + 1, 16 * wave);
-        for (int j = 0; j < 16; ++j) pn += ln[j] * xn[j];
+        for (int j = 0; j < 16; ++j) pk += kr[j] * xn[j];
     }
-    red[wave][lane] = pn;
-    __syncthreads();
-    if (wave == 0) rs[lane] = lane < nbc ? base - sred[lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]) : 0.0;
-    __syncthreads();
-    // x_b = inv(L_bb)^T rs : thread (column lane, quarter) sums 16 rows of inv
-    double pt = 0.0;
-    for (int j = 0; j < 16; ++j) pt += ivc[j] * rs[16 * wave + j];
-    __syncthreads();
-    red[wave][lane] = pt;
+    red3[wave][lane] = pk;
     __syncthreads();
     if (wave == 0) {
-        const double xv = lane < nbc ? red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane] : 0.0;
+        const double xv = lane < nbc ? (red2[0][lane] + red2[1][lane] + red2[2][lane] + red2[3][lane]) -
+                                           (red3[0][lane] + red3[1][lane] + red3[2][lane] + red3[3][lane])
+                                     : 0.0;
         if (lane < nbc) {
             v[c0 + lane] = xv;
             x[Fr.fbeg + c0 + lane] = xv;
@@ -1531,6 +1595,8 @@ void GpuCholesky<T>::prepare_inverses() {
     inv_items_ = to_device(items, stream_);
     ioff_ = to_device(ioff, stream_);
     inv_.alloc(size_t(std::max<int64_t>(o, 1)));
+    mblk_.alloc(size_t(std::max<int64_t>(o, 1)));
+    kblk_.alloc(size_t(std::max<int64_t>(o, 1)));
     n_inv_items_ = int(items.size());
 }
 
@@ -1549,9 +1615,12 @@ void GpuCholesky<T>::invert_diagonal_blocks() {
         prepare_inverses();
         inv_stale_ = false;
     }
-    if (n_inv_items_ > 0)
+    if (n_inv_items_ > 0) {
         hipLaunchKernelGGL(k_invdiag<T>, dim3(n_inv_items_), dim3(64), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
                            L_.p, inv_.p);
+        hipLaunchKernelGGL(k_chainblk<T>, dim3(n_inv_items_), dim3(256), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
+                           L_.p, inv_.p, mblk_.p, kblk_.p);
+    }
 }
 
 template <typename T>
@@ -1607,7 +1676,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                g_dst_.p + r0, g_bsrc_.p + r0, g_cptr_.p + r0, g_cidx_.p, Wr, d_b + r * ld, slots,
                                nslots, counter);
             hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p,
+                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, mblk_.p,
                                ioff_.p, Wr, d_x + r * ld);
             if (lv_l21_[l + 1] > lv_l21_[l])
                 hipLaunchKernelGGL(k_fwd_l21<T>, dim3(lv_l21_[l + 1] - lv_l21_[l]), dim3(256), 0, stream_, fronts_.p,
@@ -1636,7 +1705,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                 hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(lv_l21t_[l + 1] - lv_l21t_[l]), dim3(256), 0, stream_,
                                    fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
             hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p,
+                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, kblk_.p,
                                ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);
         }
     }

@@ -141,6 +141,7 @@ class GpuCholesky {
     DevBuf<int64_t> g_dst_, g_bsrc_, g_cidx_;
     DevBuf<int> g_xsrc_, g_cptr_;
     DevBuf<T> inv_;
+    DevBuf<T> mblk_, kblk_;  // chain shortcut blocks M_b, K_b (layout of inv_, column-major)
     int n_inv_items_ = 0;
     int max_blocks_ = 1;  // most 64-wide pivot blocks in one split-sweep level
     double factor_seconds_ = 0;
