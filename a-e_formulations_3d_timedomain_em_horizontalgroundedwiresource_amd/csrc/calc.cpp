@@ -34,6 +34,31 @@ using namespace emtd;
 
 namespace emtd {
 
+namespace {
+
+// CU mask (one bit per CU, 32 per word) of the look-ahead factorization's streams that keeps
+// EMTD_FACTOR_CU_RESERVE CUs (at most half) for the step stream alone, spread over the chip
+// (CU 37k mod n for k < reserve); empty when the variable is unset or 0.
+std::vector<uint32_t> factor_cu_mask() {
+    const char* e = std::getenv("EMTD_FACTOR_CU_RESERVE");
+    if (!e || std::atoi(e) <= 0) return {};
+    int device = 0, ncu = 0;
+    EMTD_HIP(hipGetDevice(&device));
+    EMTD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    const int reserve = std::min(std::max(0, std::atoi(e)), ncu / 2);
+    if (reserve == 0 || ncu % 37 == 0) return {};
+    std::vector<uint32_t> m((ncu + 31) / 32, 0u);
+    for (int i = 0; i < ncu; ++i) m[i / 32] |= 1u << (i % 32);
+    for (int k = 0; k < reserve; ++k) {
+        const int i = int((37LL * k) % ncu);
+        m[i / 32] &= ~(1u << (i % 32));
+    }
+    return m;
+}
+
+}  // namespace
+
+
 void keep_pool_memory(int device) {
     static std::mutex mu;
     static std::vector<int> done;
@@ -123,10 +148,15 @@ CalcCore::CalcCore(const std::string& dir, int form, int scheme, int precision, 
     if (precision_ == EMTD_FP64) {
         c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
         if (prefetch_) {
-            EMTD_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
+            const std::vector<uint32_t> mask = factor_cu_mask();
+            if (mask.empty())
+                EMTD_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
+            else
+                EMTD_HIP(hipExtStreamCreateWithCUMask(&fstream_, uint32_t(mask.size()), mask.data()));
             EMTD_HIP(hipEventCreateWithFlags(&swap_ev_, hipEventDisableTiming));
             // built on the step stream (its uploads), used on fstream_ after events
             c64b_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
+            c64b_->set_side_cu_mask(mask);
             c64b_->prepare_sweeps();
             c64_->prepare_sweeps();
         }

@@ -194,6 +194,11 @@ __global__ void __launch_bounds__(64) k_panel(const DevFront* __restrict__ F, co
     }
     if (t == 0) s_last = atomicAdd(done + f, 1) == nwg - 1;
     __syncthreads();
+    // Every load of this thread has completed here. Saying so explicitly (s_waitcnt vmcnt(0),
+    // gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15) lets the compiler issue the 64 stores below
+    // back to back: without it, its conservative view of the loads issued under per-column
+    // conditions put a vmcnt(0) wait before every store, one memory round trip each.
+    __builtin_amdgcn_s_waitcnt(0xF70);
     if (s_last) {
 #pragma unroll
         for (int j = 0; j < NB; ++j)
@@ -251,13 +256,58 @@ struct Mfma<float> {
 // 68 and the factorization from 137 to 157 ms at 1M DOF).
 constexpr int UPAD = 4;
 
+// Tile order of an update launch over ntr x ntc tiles per front (ntc <= ntr, only tiles with
+// row tile >= column tile do work): groups of 8 tile columns, each listing the rows from its
+// first column's diagonal tile down, 8 tiles of a row side by side. The workgroups of a launch
+// are dealt to the 8 XCDs in turn (workgroup w to XCD w mod 8), so w is first mapped to a
+// position s that gives every XCD one contiguous range of this list: the 64 workgroups an XCD
+// runs at once then cover about 8 tile rows x 8 tile columns and share 16 factor panels in
+// that XCD's L2, instead of about 64 panels scattered over the fronts.
+constexpr int UGROUP = 8, NXCD = 8;
+
+__host__ __device__ inline int ordered_group_tiles(int ntr, int ntc, int g) {
+    return (ntr - UGROUP * g) * min(UGROUP, ntc - UGROUP * g);
+}
+
+__host__ __device__ inline int ordered_tiles(int ntr, int ntc) {
+    int t = 0;
+    for (int g = 0; UGROUP * g < ntc; ++g) t += ordered_group_tiles(ntr, ntc, g);
+    return t;
+}
+
+// Returns false when the position is past the list (no tile).
+__device__ inline bool ordered_tile(int ntr, int ntc, int per_front, int nfronts, int* ti, int* tj, int* zi) {
+    const int64_t total = int64_t(per_front) * nfronts;
+    const int64_t w = blockIdx.x + int64_t(gridDim.x) * blockIdx.y;
+    if (w >= total) return false;
+    const int64_t q = total / NXCD, rr = total % NXCD, x = w % NXCD;
+    const int64_t s = x * q + min(x, rr) + w / NXCD;
+    *zi = int(s / per_front);
+    int loc = int(s % per_front);
+    for (int g = 0; UGROUP * g < ntc; ++g) {
+        const int n = ordered_group_tiles(ntr, ntc, g);
+        if (loc < n) {
+            const int gw = min(UGROUP, ntc - UGROUP * g);
+            *ti = UGROUP * g + loc / gw;
+            *tj = UGROUP * g + loc % gw;
+            return true;
+        }
+        loc -= n;
+    }
+    return false;
+}
+
+// ntr, ntc: row and column tiles per front; order 0: grid (ntr, ntc, fronts), order 1: grid
+// (ordered_tiles(ntr, ntc), fronts) in the order above.
 template <typename T, int TM, int WV, int KC>
 __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restrict__ F, const int* __restrict__ flist,
                                                         int k0, int k1, int cmin, int cmax, T* __restrict__ L,
-                                                        T* __restrict__ U) {
+                                                        T* __restrict__ U, int ntr, int ntc, int order) {
     constexpr int NT = 64 * WV * WV, WT = TM / WV, NA = WT / 16, E = TM * KC / NT, KS = NT / TM;
     static_assert(NT % TM == 0 && (TM * KC) % NT == 0 && WT % 16 == 0, "tile shape");
-    const DevFront Fr = F[flist[blockIdx.z]];
+    int ti = blockIdx.x, tj = blockIdx.y, zi = blockIdx.z;
+    if (order && !ordered_tile(ntr, ntc, gridDim.x, gridDim.y, &ti, &tj, &zi)) return;
+    const DevFront Fr = F[flist[zi]];
     const int p = Fr.p, m = Fr.m;
     // cmax >= 0: pivot columns [k1, min(cmax, p)) -= ...;  cmax < 0: update block U = -L21 L21^T
     const bool ublock = cmax < 0;
@@ -265,7 +315,6 @@ __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restr
     if (kend <= k0) return;
     const int cs = ublock ? p : max(kend, cmin);
     const int ce = ublock ? m : min(cmax, p);
-    const int ti = blockIdx.x, tj = blockIdx.y;
     if (ti < tj) return;
     const int R0 = cs + TM * ti, C0 = cs + TM * tj;
     if (R0 >= m || C0 >= ce) return;
@@ -332,21 +381,35 @@ __global__ void __launch_bounds__(64 * WV * WV) k_update(const DevFront* __restr
     if (skip) return;
     // acc[a][b][v] = F[R0 + wr + 16b + li][C0 + wc + 16a + row(lk, v)]
     const int u = m - p;
-    for (int a = 0; a < NA; ++a)
+    for (int a = 0; a < NA; ++a) {
+        // per column group: all reads (pivot columns), then all new values, then all writes, so
+        // that the group costs one memory round trip rather than one per entry and no write
+        // waits for an earlier one
+        T val[4][NA];
+        if (ublock) {
+            for (int v = 0; v < 4; ++v)
+                for (int b = 0; b < NA; ++b) val[v][b] = -acc[a][b][v];
+        } else {
+            // unconditional reads at clamped (in-bounds) positions: entries outside the target
+            // are read but not written
+            for (int v = 0; v < 4; ++v) {
+                const int c = min(C0 + wc + 16 * a + Mfma<T>::row(lk, v), ce - 1);
+                const T* colp = Lf + int64_t(c) * m;
+                for (int b = 0; b < NA; ++b) val[v][b] = colp[min(R0 + wr + 16 * b + li, m - 1)];
+            }
+            for (int v = 0; v < 4; ++v)
+                for (int b = 0; b < NA; ++b) val[v][b] -= acc[a][b][v];
+        }
         for (int v = 0; v < 4; ++v) {
             const int c = C0 + wc + 16 * a + Mfma<T>::row(lk, v);
             if (c >= ce) continue;
             T* colp = c < p ? L + Fr.loff + int64_t(c) * m : U + Fr.uoff + int64_t(c - p) * u - p;
             for (int b = 0; b < NA; ++b) {
                 const int r = R0 + wr + 16 * b + li;
-                if (r < m && r >= c) {
-                    if (ublock)
-                        colp[r] = -acc[a][b][v];
-                    else
-                        colp[r] -= acc[a][b][v];
-                }
+                if (r < m && r >= c) colp[r] = val[v][b];
             }
         }
+    }
 }
 
 // ------------------------------------------------------------------ triangular sweeps
@@ -1177,6 +1240,7 @@ Tuning default_tuning() {
     if (const char* e = std::getenv("EMTD_LOOKAHEAD_LDS_PAD")) t.lookahead_lds_pad = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("EMTD_SPLIT_LEVEL_FRONTS")) t.split_level_fronts = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("EMTD_SMALL_FRONT_SWEEPS")) t.small_front_sweeps = std::atoi(e) != 0;
+    if (const char* e = std::getenv("EMTD_UPDATE_ORDER")) t.update_order = std::atoi(e) != 0;
     return t;
 }
 
@@ -1284,15 +1348,18 @@ void GpuCholesky<T>::update(hipStream_t st, const int* fz, int cnt, int k0, int 
                             int cols, size_t lds_pad) {
     if (rows <= 0 || cols <= 0 || cnt <= 0) return;
     constexpr int KC = 128 / int(sizeof(T));
-    if (cols >= tune_.wide_tile_cols) {
-        const int tr = (rows + 127) / 128, tc = (cols + 127) / 128;
+    const bool wide = cols >= tune_.wide_tile_cols;
+    const int tm = wide ? 128 : 64, tr = (rows + tm - 1) / tm, tc = (cols + tm - 1) / tm;
+    // the ordered grid lists only tiles on or below the diagonal, which needs tc <= tr
+    const int order = tune_.update_order && tc <= tr ? 1 : 0;
+    const dim3 grid = order ? dim3(ordered_tiles(tr, tc), cnt) : dim3(tr, tc, cnt);
+    if (wide) {
         const size_t pad = std::max(lds_pad, size_t(tune_.update_lds_pad));
-        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), dim3(tr, tc, cnt), dim3(1024), pad, st, fronts_.p, fz, k0,
-                           k1, cmin, cmax, L_.p, U_.p);
+        hipLaunchKernelGGL((k_update<T, 128, 4, KC>), grid, dim3(1024), pad, st, fronts_.p, fz, k0, k1, cmin, cmax,
+                           L_.p, U_.p, tr, tc, order);
     } else {
-        const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
-        hipLaunchKernelGGL((k_update<T, 64, 2, KC>), dim3(tr, tc, cnt), dim3(256), 0, st, fronts_.p, fz, k0, k1, cmin,
-                           cmax, L_.p, U_.p);
+        hipLaunchKernelGGL((k_update<T, 64, 2, KC>), grid, dim3(256), 0, st, fronts_.p, fz, k0, k1, cmin, cmax, L_.p,
+                           U_.p, tr, tc, order);
     }
 }
 
@@ -1316,7 +1383,10 @@ template <typename T>
 void GpuCholesky<T>::numeric() {
     const Symbolic& S = S_;
     if (!side_) {
-        EMTD_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        if (side_mask_.empty())
+            EMTD_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        else
+            EMTD_HIP(hipExtStreamCreateWithCUMask(&side_, uint32_t(side_mask_.size()), side_mask_.data()));
         EMTD_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
         EMTD_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     }
@@ -1353,8 +1423,9 @@ void GpuCholesky<T>::numeric() {
                     const int rem = mm - ks - 1;
                     hipLaunchKernelGGL(k_panel<T>, dim3(std::max(1, (rem + 63) / 64), cnt), dim3(64), 0, stream_,
                                        fronts_.p, fz, ks, L_.p, info_.p, panel_done_.p);
-                    // inside the outer block: columns [ks+NB, kb+nbo) with K = NB
-                    update(stream_, fz, cnt, ks, ks + NB, 0, kb + nbo, rem, std::min(kb + nbo, mp) - (ks + 1));
+                    // inside the outer block: columns [ks+NB, kb+nbo) with K = NB, rows from ks+NB
+                    // (none after the block's last panel)
+                    update(stream_, fz, cnt, ks, ks + NB, 0, kb + nbo, mm - (ks + NB), std::min(kb + nbo, mp) - (ks + NB));
                 }
                 // the previous bulk update wrote the columns the trailing updates touch
                 join_side();

@@ -43,15 +43,18 @@ struct Tuning {
     int small_front_sweeps = 1;     // one wavefront per front where every front has <= 64 pivots
     int update_lds_pad = 0;         // extra LDS bytes of every 128x128-tile update launch (room on
                                     // each CU for another stream's kernels during a look-ahead)
+    int update_order = 0;           // 1: update tiles in 8-column groups, one contiguous range per XCD
     bool operator==(const Tuning& o) const {
         return outer_block == o.outer_block && wide_tile_cols == o.wide_tile_cols &&
                lookahead_lds_pad == o.lookahead_lds_pad && split_level_fronts == o.split_level_fronts &&
-               small_front_sweeps == o.small_front_sweeps && update_lds_pad == o.update_lds_pad;
+               small_front_sweeps == o.small_front_sweeps && update_lds_pad == o.update_lds_pad &&
+               update_order == o.update_order;
     }
 };
 
 // The defaults, with the environment overrides EMTD_OUTER_BLOCK, EMTD_WIDE_TILE_COLS,
-// EMTD_LOOKAHEAD_LDS_PAD, EMTD_SPLIT_LEVEL_FRONTS and EMTD_SMALL_FRONT_SWEEPS applied.
+// EMTD_LOOKAHEAD_LDS_PAD, EMTD_SPLIT_LEVEL_FRONTS, EMTD_SMALL_FRONT_SWEEPS and
+// EMTD_UPDATE_ORDER applied.
 Tuning default_tuning();
 
 // Device copy of the symbolic structure plus factor storage of type T (double or float).
@@ -78,6 +81,9 @@ class GpuCholesky {
     // Stream of the solver's factorization and solves (a factor can be built on one stream and
     // used on another).
     void set_stream(hipStream_t s) { stream_ = s; }
+    // CU mask of the factorization's second stream (empty: all CUs); takes effect when that stream
+    // is created, at the first factorization
+    void set_side_cu_mask(const std::vector<uint32_t>& m) { side_mask_ = m; }
     // Numeric factorization of a given lower RSF matrix (ggl, di) on the device.
     void factorize_values(const int64_t* d_dest_off, const int64_t* d_dest_diag, int64_t nnz_low, int n,
                           const double* d_ggl, const double* d_di);
@@ -112,6 +118,7 @@ class GpuCholesky {
     const Symbolic& S_;
     hipStream_t stream_;
     hipStream_t side_ = nullptr;  // second stream of the factorization (look-ahead)
+    std::vector<uint32_t> side_mask_;
     hipEvent_t ev_main_ = nullptr, ev_side_ = nullptr;
     DevBuf<T> L_, U_;
     DevBuf<double> W_;

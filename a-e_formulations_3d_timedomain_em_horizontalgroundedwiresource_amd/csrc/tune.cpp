@@ -24,10 +24,10 @@ std::map<std::string, Tuning>& cache() {
 }
 
 std::string describe(const Tuning& t) {
-    char b[160];
-    std::snprintf(b, sizeof b, "outer_block=%d wide_tile_cols=%d lookahead_lds_pad=%d split_level_fronts=%d "
-                  "small_front_sweeps=%d", t.outer_block, t.wide_tile_cols, t.lookahead_lds_pad,
-                  t.split_level_fronts, t.small_front_sweeps);
+    char b[192];
+    std::snprintf(b, sizeof b, "outer_block=%d wide_tile_cols=%d lookahead_lds_pad=%d update_order=%d "
+                  "split_level_fronts=%d small_front_sweeps=%d", t.outer_block, t.wide_tile_cols,
+                  t.lookahead_lds_pad, t.update_order, t.split_level_fronts, t.small_front_sweeps);
     return b;
 }
 
@@ -98,6 +98,11 @@ Tuning autotune(GpuCholesky<T>& C, const std::function<bool()>& refactor, const 
     {
         Tuning t = best;
         t.lookahead_lds_pad = best.lookahead_lds_pad ? 0 : 48 * 1024;
+        try_factor(t);
+    }
+    {
+        Tuning t = best;
+        t.update_order = !best.update_order;
         try_factor(t);
     }
     C.set_tuning(best);

@@ -12,7 +12,8 @@ namespace emtd {
 
 // Coordinate descent over the launch parameters, starting from the solver's current tuning:
 // factorization parameters first (outer block 64/128/256, 128x128 tiles from 128/256/512 target
-// columns, look-ahead LDS padding on/off), each candidate timed by refactoring the same matrix
+// columns, look-ahead LDS padding on/off, XCD-grouped update tile order on/off), each candidate
+// timed by refactoring the same matrix
 // (refactor() returns false on a pivot failure, which rejects the candidate); then the sweep
 // strategy (split-sweep threshold 0/256/1024/4096 fronts per level, one-wavefront-per-front
 // sweeps on/off), each timed over three solves of b into x. A candidate replaces the best one
