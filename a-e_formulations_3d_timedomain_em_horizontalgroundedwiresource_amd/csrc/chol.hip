@@ -925,14 +925,17 @@ __global__ void __launch_bounds__(64) k_invdiag(const DevFront* __restrict__ F, 
 }
 
 // The links' shortcut blocks, per pivot block b of a split-sweep front (after k_invdiag):
-//   M_b = inv(L_bb) L_{b,b-1}        (b >= 1; forward:  y_b = inv(L_bb)(v_b - ...) - M_b y_{b-1})
-//   K_b = inv(L_bb)^T L_{b+1,b}^T    (b + 1 < np; backward: x_b = inv(L_bb)^T(...) - K_b x_{b+1})
-// both stored column-major ([col][row], so a wavefront reads a column with consecutive lanes),
+//   M_b  = inv(L_bb) L_{b,b-1}        (b >= 1; forward:  y_b = inv(L_bb)(v_b - ...) - M_b y_{b-1}
+//   M2_b = inv(L_bb) L_{b,b-2}        (b >= 2;                                - M2_b y_{b-2})
+//   K_b  = inv(L_bb)^T L_{b+1,b}^T    (b + 1 < np; backward: x_b = inv(L_bb)^T(...) - K_b x_{b+1}
+//   K2_b = inv(L_bb)^T L_{b+2,b}^T     (b + 2 < np;                               - K2_b x_{b+2})
+// all stored column-major ([col][row], so a wavefront reads a column with consecutive lanes),
 // zero outside the valid rows/columns.
 template <typename T>
 __global__ void __launch_bounds__(256) k_chainblk(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                   const int64_t* __restrict__ ioff, const T* __restrict__ L,
-                                                  const T* __restrict__ inv, T* __restrict__ mblk, T* __restrict__ kblk) {
+                                                  const T* __restrict__ inv, T* __restrict__ mblk, T* __restrict__ m2blk,
+                                                  T* __restrict__ kblk, T* __restrict__ k2blk) {
     __shared__ double A[SB][SB + 1];  // inv(L_bb), [row][col]
     __shared__ double Bm[SB][SB + 1];
     const int2 it = items[blockIdx.x];
@@ -960,6 +963,20 @@ __global__ void __launch_bounds__(256) k_chainblk(const DevFront* __restrict__ F
         mblk[base + int64_t(j) * SB + i] = T(acc);
     }
     __syncthreads();
+    // M2_b[i][j] = sum_k inv[i][k] L[64b + k][64(b-2) + j]
+    for (int e = t; e < SB * SB; e += 256) {
+        const int k = e % SB, j = e / SB;
+        Bm[k][j] = (b >= 2 && k < nb) ? double(Lf[(64 * (b - 2) + j) * m + 64 * b + k]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < SB * SB; e += 256) {
+        const int i = e % SB, j = e / SB;
+        double acc = 0.0;
+        if (b >= 2)
+            for (int k = 0; k < SB; ++k) acc += A[i][k] * Bm[k][j];
+        m2blk[base + int64_t(j) * SB + i] = T(acc);
+    }
+    __syncthreads();
     // K_b[i][j] = sum_k inv[k][i] L[64(b+1) + j][64b + k]
     const int nb1 = b + 1 < np ? min(64, p - 64 * (b + 1)) : 0;
     for (int e = t; e < SB * SB; e += 256) {
@@ -974,15 +991,35 @@ __global__ void __launch_bounds__(256) k_chainblk(const DevFront* __restrict__ F
             for (int k = 0; k < SB; ++k) acc += A[k][i] * Bm[k][j];
         kblk[base + int64_t(j) * SB + i] = T(acc);
     }
+    __syncthreads();
+    // K2_b[i][j] = sum_k inv[k][i] L[64(b+2) + j][64b + k]
+    const int nb2 = b + 2 < np ? min(64, p - 64 * (b + 2)) : 0;
+    for (int e = t; e < SB * SB; e += 256) {
+        const int j = e % SB, k = e / SB;
+        Bm[k][j] = (j < nb2 && k < nb) ? double(Lf[(64 * b + k) * m + 64 * (b + 2) + j]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < SB * SB; e += 256) {
+        const int i = e % SB, j = e / SB;
+        double acc = 0.0;
+        if (nb2 > 0)
+            for (int k = 0; k < SB; ++k) acc += A[k][i] * Bm[k][j];
+        k2blk[base + int64_t(j) * SB + i] = T(acc);
+    }
 }
 
+// Forward chain link for pivot row block rb of a front. Before any wait it loads its inverse
+// block and shortcut blocks and its gathered right-hand side; it accumulates
+// t = v_b - sum_{c < b-2} L_bc y_c as the earlier blocks arrive and forms w = inv(L_bb) t
+// while blocks b-2 and b-1 are still in flight; after them only
+// y_b = w - M2_b y_{b-2} - M_b y_{b-1} remains (16 products per thread each, one LDS reduction).
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ ybuf, int* __restrict__ counter,
                                                    SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const T* __restrict__ mblk,
-                                                   const int64_t* __restrict__ ioff, double* __restrict__ W,
-                                                   double* __restrict__ x) {
+                                                   const T* __restrict__ m2blk, const int64_t* __restrict__ ioff,
+                                                   double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
     __shared__ double red[4][64], red2[4][64], red3[4][64];
     __shared__ double rs[64];
@@ -998,17 +1035,18 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     double* yb = ybuf + int64_t(foff[it.x]) * 64;
-    // loaded before any wait: this thread's 16 entries of inv(L_bb) and of M_b = inv(L_bb) L_{b,b-1}
-    // (row lr, columns 16q..16q+15) and the gathered right-hand side of its row
+    // this thread's 16 entries of inv(L_bb), M_b and M2_b (row lr, columns 16q..16q+15) and the
+    // gathered right-hand side of its row
     const int64_t boff = ioff[it.x] + int64_t(rb) * SB * SB;
     const T* iv = inv + boff + lr * SB + 16 * q;
-    double ivr[16], mr[16];
+    double ivr[16], mr[16], m2r[16];
     for (int j = 0; j < 16; ++j) ivr[j] = double(iv[j]);
     for (int j = 0; j < 16; ++j) mr[j] = rb > 0 ? double(mblk[boff + int64_t(16 * q + j) * SB + lr]) : 0.0;
+    for (int j = 0; j < 16; ++j) m2r[j] = rb > 1 ? double(m2blk[boff + int64_t(16 * q + j) * SB + lr]) : 0.0;
     const double vr = (q == 0 && lr < nbp) ? v[r] : 0.0;
-    // t = v_b - sum_{c < b-1} L_bc y_c as the earlier blocks arrive
+    // t = v_b - sum_{c < b-2} L_bc y_c as the earlier blocks arrive
     double acc = 0.0;
-    for (int cb = 0; cb + 1 < rb; ++cb) {
+    for (int cb = 0; cb + 2 < rb; ++cb) {
         double lv[16], yv[16];
         const T* lc = Lf + int64_t(64 * cb + 16 * q) * m + r;
         for (int j = 0; j < 16; ++j) lv[j] = lr < nbp ? double(lc[int64_t(j) * m]) : 0.0;
@@ -1023,8 +1061,13 @@ __global__ void __launch_bounds__(256) k_fwd_chain(const DevFront* __restrict__ 
     double part = 0.0;
     for (int j = 0; j < 16; ++j) part += ivr[j] * rs[16 * q + j];
     red2[q][lr] = part;
-    // y_b = w - M_b y_{b-1}: after the previous block's result only this product remains
+    // y_b = w - M2_b y_{b-2} - M_b y_{b-1}
     double pm = 0.0;
+    if (rb > 1) {
+        double yv[16];
+        poll16(yb + 64 * (rb - 2) + 16 * q, yv, ws, rb - 2, 16 * q);
+        for (int j = 0; j < 16; ++j) pm += m2r[j] * yv[j];
+    }
     if (rb > 0) {
         double yv[16];
         poll16(yb + 64 * (rb - 1) + 16 * q, yv, ws, rb - 1, 16 * q);
@@ -1135,17 +1178,18 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
 }
 
 // Backward chain link for pivot column block cb of a front. Everything that does not depend on
-// the next block's result is done before waiting for it: the partial sums of L21^T x_U and the
-// gathered y_b, the blocks cb2 > cb+1 (coalesced reads, lane = row, reduced over lanes once),
-// and z = inv(L_bb)^T (y_b - t_b - sum_{cb2 > cb+1} L_{cb2,cb}^T x_cb2); after the next block's
-// result only x_b = z - K_b x_{cb+1} remains (K_b = inv(L_bb)^T L_{cb+1,cb}^T, 16 products per
-// thread and one LDS reduction).
+// the two next blocks' results is done before waiting for them: the partial sums of L21^T x_U
+// and the gathered y_b, the blocks cb2 > cb+2 (coalesced reads, lane = row, reduced over lanes
+// once), and z = inv(L_bb)^T (y_b - t_b - sum_{cb2 > cb+2} L_{cb2,cb}^T x_cb2). After them only
+// x_b = z - K2_b x_{cb+2} - K_b x_{cb+1} remains (K_b = inv(L_bb)^T L_{cb+1,cb}^T,
+// K2_b = inv(L_bb)^T L_{cb+2,cb}^T: 16 products per thread each, one LDS reduction), so the
+// lane reduction of the far blocks runs while the link waits rather than after a wait.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ F, const int2* __restrict__ items,
                                                    const int* __restrict__ foff, double* __restrict__ xbuf, int* __restrict__ counter,
                                                    SweepDiag* __restrict__ dg, int level, const T* __restrict__ L,
                                                    const T* __restrict__ inv, const T* __restrict__ kblk,
-                                                   const int64_t* __restrict__ ioff,
+                                                   const T* __restrict__ k2blk, const int64_t* __restrict__ ioff,
                                                    const double* __restrict__ part, const int64_t* __restrict__ poff,
                                                    double* __restrict__ W, double* __restrict__ x) {
     __shared__ int s_ticket;
@@ -1164,14 +1208,15 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     const T* Lf = L + Fr.loff;
     double* v = W + Fr.woff;
     double* xb = xbuf + int64_t(foff[it.x]) * 64;
-    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15) and of K_b (row lane,
-    // columns 16w..16w+15)
+    // this thread's 16 entries of inv(L_bb) (column lane, rows 16w..16w+15) and of K_b and K2_b
+    // (row lane, columns 16w..16w+15)
     const int64_t boff = ioff[it.x] + int64_t(cb) * SB * SB;
     const T* iv = inv + boff + lane;
-    const bool has_next = cb + 1 < np;
-    double ivc[16], kr[16];
+    const bool has_next = cb + 1 < np, has_next2 = cb + 2 < np;
+    double ivc[16], kr[16], k2r[16];
     for (int j = 0; j < 16; ++j) ivc[j] = double(iv[(16 * wave + j) * SB]);
     for (int j = 0; j < 16; ++j) kr[j] = has_next ? double(kblk[boff + int64_t(16 * wave + j) * SB + lane]) : 0.0;
+    for (int j = 0; j < 16; ++j) k2r[j] = has_next2 ? double(k2blk[boff + int64_t(16 * wave + j) * SB + lane]) : 0.0;
     // y_b - L21^T x_U for this thread's column (wave 0)
     double base = 0.0;
     if (wave == 0 && lane < nbc) {
@@ -1180,10 +1225,10 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
         for (int k = 0; k < nchunk; ++k) t += pp[int64_t(k) * p];
         base = v[c0 + lane] - t;
     }
-    // blocks cb2 = np-1 .. cb+2
+    // blocks cb2 = np-1 .. cb+3
     double acc[16];
     for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-    for (int cb2 = np - 1; cb2 > cb + 1; --cb2) {
+    for (int cb2 = np - 1; cb2 > cb + 2; --cb2) {
         const int nb2 = min(64, p - 64 * cb2);
         double lv[16];
         for (int j = 0; j < 16; ++j) {
@@ -1205,8 +1250,13 @@ __global__ void __launch_bounds__(256) k_bwd_chain(const DevFront* __restrict__ 
     double pt = 0.0;
     for (int j = 0; j < 16; ++j) pt += ivc[j] * rs[16 * wave + j];
     red2[wave][lane] = pt;
-    // x_b = z - K_b x_{cb+1} (its rows past the block end are published as 0)
+    // x_b = z - K2_b x_{cb+2} - K_b x_{cb+1} (its rows past the block end are published as 0)
     double pk = 0.0;
+    if (has_next2) {
+        double xn[16];
+        poll16(xb + 64 * (cb + 2) + 16 * wave, xn, ws2, cb + 2, 16 * wave);
+        for (int j = 0; j < 16; ++j) pk += k2r[j] * xn[j];
+    }
     if (has_next) {
         double xn[16];
         poll16(xb + 64 * (cb + 1) + 16 * wave, xn, ws3, cb + 1, 16 * wave);
@@ -1667,7 +1717,9 @@ void GpuCholesky<T>::prepare_inverses() {
     ioff_ = to_device(ioff, stream_);
     inv_.alloc(size_t(std::max<int64_t>(o, 1)));
     mblk_.alloc(size_t(std::max<int64_t>(o, 1)));
+    m2blk_.alloc(size_t(std::max<int64_t>(o, 1)));
     kblk_.alloc(size_t(std::max<int64_t>(o, 1)));
+    k2blk_.alloc(size_t(std::max<int64_t>(o, 1)));
     n_inv_items_ = int(items.size());
 }
 
@@ -1690,7 +1742,7 @@ void GpuCholesky<T>::invert_diagonal_blocks() {
         hipLaunchKernelGGL(k_invdiag<T>, dim3(n_inv_items_), dim3(64), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
                            L_.p, inv_.p);
         hipLaunchKernelGGL(k_chainblk<T>, dim3(n_inv_items_), dim3(256), 0, stream_, fronts_.p, inv_items_.p, ioff_.p,
-                           L_.p, inv_.p, mblk_.p, kblk_.p);
+                           L_.p, inv_.p, mblk_.p, m2blk_.p, kblk_.p, k2blk_.p);
     }
 }
 
@@ -1747,7 +1799,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                                g_dst_.p + r0, g_bsrc_.p + r0, g_cptr_.p + r0, g_cidx_.p, Wr, d_b + r * ld, slots,
                                nslots, counter);
             hipLaunchKernelGGL(k_fwd_chain<T>, dim3(lv_fc_[l + 1] - lv_fc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, mblk_.p,
+                               fc_items_.p + lv_fc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, mblk_.p, m2blk_.p,
                                ioff_.p, Wr, d_x + r * ld);
             if (lv_l21_[l + 1] > lv_l21_[l])
                 hipLaunchKernelGGL(k_fwd_l21<T>, dim3(lv_l21_[l + 1] - lv_l21_[l]), dim3(256), 0, stream_, fronts_.p,
@@ -1776,7 +1828,7 @@ void GpuCholesky<T>::solve(const double* d_b, double* d_x, int nrhs, int64_t ld)
                 hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(lv_l21t_[l + 1] - lv_l21t_[l]), dim3(256), 0, stream_,
                                    fronts_.p, l21t_items_.p + lv_l21t_[l], poff_.p, L_.p, Wr, part_.p);
             hipLaunchKernelGGL(k_bwd_chain<T>, dim3(lv_bc_[l + 1] - lv_bc_[l]), dim3(256), 0, stream_, fronts_.p,
-                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, kblk_.p,
+                               bc_items_.p + lv_bc_[l], foff_.p, xbuf_.p, counter, dg, l, L_.p, inv_.p, kblk_.p, k2blk_.p,
                                ioff_.p, part_.p, poff_.p, Wr, d_x + r * ld);
         }
     }

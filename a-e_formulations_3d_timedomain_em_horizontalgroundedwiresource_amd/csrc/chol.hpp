@@ -148,7 +148,8 @@ class GpuCholesky {
     DevBuf<int64_t> g_dst_, g_bsrc_, g_cidx_;
     DevBuf<int> g_xsrc_, g_cptr_;
     DevBuf<T> inv_;
-    DevBuf<T> mblk_, kblk_;  // chain shortcut blocks M_b, K_b (layout of inv_, column-major)
+    DevBuf<T> mblk_, m2blk_, kblk_, k2blk_;  // chain shortcut blocks M_b, M2_b, K_b, K2_b (layout of inv_,
+                                             // column-major)
     int n_inv_items_ = 0;
     int max_blocks_ = 1;  // most 64-wide pivot blocks in one split-sweep level
     double factor_seconds_ = 0;
