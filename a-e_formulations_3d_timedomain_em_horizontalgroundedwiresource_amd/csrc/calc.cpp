@@ -377,7 +377,9 @@ CalcCore::~CalcCore() {
 int64_t CalcCore::estimate_footprint() const {
     const Symbolic& S = *S_;
     const int64_t n = n_, nnz = nnz_, np = npls_, nn = nn_;
-    auto factor = [&](int64_t elem) { return (S.lsize + S.usize + nn * 64) * elem + S.wsize * 8 * np + nn * 160; };
+    // factor, update pool, and at most 64 entries per row in each of the five per-pivot-block
+    // arrays of the sweeps (inverted diagonal blocks and the chain shortcut blocks)
+    auto factor = [&](int64_t elem) { return (S.lsize + S.usize + nn * 64 * 5) * elem + S.wsize * 8 * np + nn * 160; };
     int64_t b = 2 * (nnz + n) * 8;        // B and C values
     b += (nnz + n) * 8;                   // scatter maps into the factor
     b += (2 * nnz + n) * 12 + n * 8;      // C in permuted CSR
