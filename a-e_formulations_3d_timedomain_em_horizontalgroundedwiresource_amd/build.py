@@ -16,8 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemtd.so")
 EXE = os.path.join(HERE, "calcnonstat3dline")
-# drop-in executables: CalcNonstat3DLine, SumNonstat3DOnly, SummatorEDMN
-MAINS = {"calc_main.cpp": "calcnonstat3dline", "sum_main.cpp": "sumnonstat3donly", "edmn_main.cpp": "summatoredmn"}
+# drop-in executables: CalcNonstat3DLine, SumNonstat3DOnly, SummatorEDMN, TimeDirectSolver, CorrectTimeMesh
+MAINS = {"calc_main.cpp": "calcnonstat3dline", "sum_main.cpp": "sumnonstat3donly", "edmn_main.cpp": "summatoredmn",
+         "timemesh_main.cpp": "timedirectsolver", "correct_main.cpp": "correcttimemesh"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -1,0 +1,270 @@
+// Time-mesh preparation before the calculation (host code, no GPU).
+//
+// emtd_time_direct_solver replaces TimeDirectSolver (M/TimeDirectSolver/main.cpp:98-219 with
+// TimeStep::GenRegMesh and TimeStep::WriteTimes, M/TimeDirectSolver/TimeStep.cpp:76-171): the
+// user's time points in infite.0 become a grid of piecewise constant steps that grow by the
+// factor in tcff from one interval to the next, and the source tables currentfunction and
+// deltafunction are resampled on it by linear interpolation (held constant outside).
+//
+// emtd_correct_time_mesh replaces CorrectTimeMesh (M/CorrectTimeMesh/main.cpp:87-266): a decade
+// (run of equal steps) that is too short compared with the one before it, i.e. whose length
+// times tcff does not exceed the previous decade's length, is replaced by a continuation of
+// the previous decade's mean step; deltafunction is resampled on the new grid and written to
+// both deltafunction and currentfunction.
+//
+// Both keep the inputs under the reference's backup names (!infite.0, !currentfunction,
+// !deltafunction; !!... for CorrectTimeMesh). Numbers are written as the reference's iostreams
+// write them with std::scientific: times in infite.0 with 7 digits after the point, tables
+// with 14.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/emtd.h"
+#include "common.hpp"
+#include "model_io.hpp"
+
+// the interpolation below is evaluated as written (no fused multiply-add), like the oracle
+#pragma STDC FP_CONTRACT OFF
+
+using namespace emtd;
+
+namespace {
+
+std::vector<double> numbers(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw Error(EMTD_ERR_NOFILE, "cannot open " + path);
+    std::vector<double> v;
+    for (double x; in >> x;) v.push_back(x);
+    return v;
+}
+
+double read_tcff(const std::string& dir) {
+    const std::vector<double> v = numbers(join_path(dir, "tcff"));
+    if (v.empty()) throw Error(EMTD_ERR_BADFILE, "bad tcff");
+    return v[0];
+}
+
+// FindIntervalInDoubleVec (M/TimeDirectSolver/main.cpp:43-57): bisection for the interval
+// [vec[i], vec[i+1]] that holds elem (vec[0] <= elem <= vec.back()).
+int find_interval(const std::vector<double>& vec, double elem) {
+    int i = 0, k = int(vec.size()) - 1;
+    if (!k || elem < vec[0] || elem > vec[k]) return -1;
+    int j = (i + k) / 2;
+    do {
+        if (elem > vec[j])
+            i = j;
+        else
+            k = j;
+        j = (i + k) / 2;
+    } while (j != i);
+    return i;
+}
+
+// Interpolate (M/TimeDirectSolver/main.cpp:74-96): linear inside the table, end values outside.
+double interpolate(double t, const std::vector<double>& vt, const std::vector<double>& vv) {
+    const int last = int(vt.size()) - 1;
+    if (t < vt[0]) return vv[0];
+    if (t > vt[last]) return vv[last];
+    const int it = find_interval(vt, t);
+    const double t1 = vt[it], t2 = vt[it + 1];
+    return ((t2 - t) * vv[it] + (t - t1) * vv[it + 1]) / (t2 - t1);
+}
+
+std::vector<double> resample(const std::vector<double>& t_new, const std::vector<double>& vt,
+                             const std::vector<double>& vv) {
+    if (vt.size() < 2) throw Error(EMTD_ERR_BADFILE, "a source table needs at least two rows");
+    std::vector<double> out(t_new.size());
+    for (size_t i = 0; i < t_new.size(); ++i) out[i] = interpolate(t_new[i], vt, vv);
+    return out;
+}
+
+// TimeStep::GenRegMesh (M/TimeDirectSolver/TimeStep.cpp:76-143). An interval of constant step
+// ends at the first point whose step exceeds coef times the current step, or at the last
+// point; it gets floor((t_k - start) / step) + 1 steps, and the next interval's step is coef
+// times larger.
+std::vector<double> gen_reg_mesh(const std::vector<double>& t, double coef) {
+    if (t.size() < 3) throw Error(EMTD_ERR_BADFILE, "infite.0 needs at least three times");
+    double step_old = t[1] - t[0];
+    std::vector<double> bounds{t[0]}, h{step_old};
+    std::vector<int> nsub;
+    for (size_t k = 2; k < t.size(); ++k) {
+        const double step = t[k] - t[k - 1];
+        if (step > coef * step_old || k + 1 == t.size()) {
+            nsub.push_back(int((t[k] - bounds.back()) / step_old + 1));
+            bounds.push_back(bounds.back() + nsub.back() * step_old);
+            step_old *= coef;
+            h.push_back(step_old);
+        }
+    }
+    std::vector<double> g{t[0]};
+    for (size_t i = 0; i + 1 < bounds.size(); ++i)
+        for (int j = 0; j < nsub[i]; ++j) g.push_back(bounds[i] + (j + 1) * h[i]);
+    // the reference's trailing trim: while the last point but one lies past the last input
+    // time, the last point goes
+    while (g.size() >= 2 && g[g.size() - 2] > t.back()) g.pop_back();
+    return g;
+}
+
+std::string sci(double v, int digits) {
+    char b[48];
+    std::snprintf(b, sizeof b, "%.*e", digits, v);
+    return b;
+}
+
+void write_text(const std::string& path, const std::string& s) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw Error(EMTD_ERR_NOFILE, "cannot write " + path);
+    f << s;
+}
+
+// TimeStep::WriteTimes (M/TimeDirectSolver/TimeStep.cpp:144-171), also SaveTimeMesh's infite.0
+// (M/CorrectTimeMesh/main.cpp:150-176): header lines, then five times per line.
+void write_infite0(const std::string& path, const std::vector<double>& g) {
+    const int nt = int(g.size());
+    std::string s = "    ktime=  " + std::to_string(nt) + ";  ntstop=  " + std::to_string(nt) +
+                    ";   kiter=    1;   ntime=    1;   niter=    1;\n"
+                    "   kprogm=    1;  kpropi=    1;  kpropt=   -1;  kitrel=  250;              ;\n"
+                    "       u0=   0.00000e+000\n"
+                    " T I M E :\n";
+    for (int i = 0; i < nt; i += 5) {
+        for (int j = 0; j < 5 && i + j < nt; ++j) s += " " + sci(g[i + j], 7) + ";";
+        s += "\n";
+    }
+    write_text(path, s);
+}
+
+std::string table_rows(const std::vector<double>& t, const std::vector<double>& v) {
+    std::string s;
+    for (size_t i = 0; i < t.size(); ++i) s += sci(t[i], 14) + " " + sci(v[i], 14) + "\n";
+    return s;
+}
+
+struct DeltaTable {
+    double recrad = 0;
+    std::vector<double> t, v;
+};
+
+// deltafunction: the receiver radius, the row count, then (t, value) rows
+DeltaTable read_delta(const std::string& path) {
+    const std::vector<double> x = numbers(path);
+    if (x.size() < 2) throw Error(EMTD_ERR_BADFILE, "bad deltafunction");
+    DeltaTable d;
+    d.recrad = x[0];
+    const double n = x[1];
+    if (!(n >= 0) || 2 + 2 * size_t(n) > x.size()) throw Error(EMTD_ERR_BADFILE, "truncated deltafunction");
+    for (size_t i = 0; i < size_t(n); ++i) {
+        d.t.push_back(x[2 + 2 * i]);
+        d.v.push_back(x[3 + 2 * i]);
+    }
+    return d;
+}
+
+void write_delta(const std::string& path, double recrad, const std::vector<double>& t, const std::vector<double>& v) {
+    write_text(path, sci(recrad, 14) + "\n" + std::to_string(t.size()) + "\n" + table_rows(t, v));
+}
+
+// GenerateDecInfo (M/CorrectTimeMesh/main.cpp:87-104): decade starts, where the step changes by
+// more than 1e-3 relative, closed by ntime.
+std::vector<int> decade_starts(const std::vector<double>& t) {
+    std::vector<int> ig{0};
+    double hpre = t[1] - t[0];
+    for (size_t i = 2; i < t.size(); ++i) {
+        const double hcur = t[i] - t[i - 1];
+        if (std::fabs(1.0 - hpre / hcur) > 1e-3) {
+            ig.push_back(int(i));
+            hpre = hcur;
+        }
+    }
+    ig.push_back(int(t.size()));
+    return ig;
+}
+
+// The corrected grid of CorrectTimeMesh's main (M/CorrectTimeMesh/main.cpp:231-263).
+std::vector<double> correct_mesh(const std::vector<double>& t, double tcff) {
+    if (t.size() < 2) throw Error(EMTD_ERR_BADFILE, "infite.0 needs at least two times");
+    const std::vector<int> ig = decade_starts(t);
+    const int ndec = int(ig.size()) - 1;
+    std::vector<double> out;
+    for (int d = 0; d < ndec; ++d) {
+        const int len = ig[d + 1] - ig[d];
+        if (len > 1 && (d == 0 || len * tcff > ig[d] - ig[d - 1])) {
+            out.insert(out.end(), t.begin() + ig[d], t.begin() + ig[d + 1]);
+            continue;
+        }
+        if (d == 0) throw Error(EMTD_ERR_BADFILE, "error in first decade");
+        // continue with the previous decade's mean step (the quotient is NaN or infinite when
+        // that decade has a single point, and then no point is added, as in the reference)
+        const double base = t[ig[d] - 1];
+        const double hpre = (base - t[ig[d - 1]]) / double(ig[d] - ig[d - 1] - 1);
+        for (int i = 1;; ++i) {
+            const double tcur = base + i * hpre;
+            if (!(tcur - 0.5 * hpre < t[ig[d + 1] - 1])) break;
+            out.push_back(tcur);
+        }
+    }
+    return out;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return EMTD_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return EMTD_ERR_ERROR;
+    }
+}
+
+}  // namespace
+
+extern "C" int emtd_time_direct_solver(const char* workdir) {
+    return guarded([&] {
+        if (!workdir) throw Error(EMTD_ERR_RANGE, "null argument");
+        const std::string d = workdir;
+        auto P = [&](const char* s) { return join_path(d, s); };
+        const double tcff = read_tcff(d);
+        const std::vector<double> times = read_time_grid(P("infite.0"));
+        const std::vector<double> g = gen_reg_mesh(times, tcff);
+        // every input is read before anything is renamed or written
+        const std::vector<double> cf = numbers(P("currentfunction"));
+        std::vector<double> ct, cv;
+        for (size_t i = 0; i + 1 < cf.size(); i += 2) {
+            ct.push_back(cf[i]);
+            cv.push_back(cf[i + 1]);
+        }
+        const DeltaTable dt = read_delta(P("deltafunction"));
+        const std::vector<double> cnew = resample(g, ct, cv), dnew = resample(g, dt.t, dt.v);
+        std::rename(P("infite.0").c_str(), P("!infite.0").c_str());
+        write_infite0(P("infite.0"), g);
+        std::rename(P("currentfunction").c_str(), P("!currentfunction").c_str());
+        write_text(P("currentfunction"), table_rows(g, cnew));
+        std::rename(P("deltafunction").c_str(), P("!deltafunction").c_str());
+        write_delta(P("deltafunction"), dt.recrad, g, dnew);
+    });
+}
+
+extern "C" int emtd_correct_time_mesh(const char* workdir) {
+    return guarded([&] {
+        if (!workdir) throw Error(EMTD_ERR_RANGE, "null argument");
+        const std::string d = workdir;
+        auto P = [&](const char* s) { return join_path(d, s); };
+        const double tcff = read_tcff(d);
+        const std::vector<double> times = read_time_grid(P("infite.0"));
+        const std::vector<double> g = correct_mesh(times, tcff);
+        const DeltaTable dt = read_delta(P("deltafunction"));
+        const std::vector<double> v = resample(g, dt.t, dt.v);
+        std::rename(P("currentfunction").c_str(), P("!!currentfunction").c_str());
+        std::rename(P("infite.0").c_str(), P("!!infite.0").c_str());
+        std::rename(P("deltafunction").c_str(), P("!!deltafunction").c_str());
+        write_delta(P("deltafunction"), dt.recrad, g, v);
+        write_text(P("currentfunction"), table_rows(g, v));
+        write_infite0(P("infite.0"), g);
+    });
+}

@@ -177,6 +177,21 @@ int emtd_sum_nonstat_3d_only(const char* workdir);
  * when the file xyzmn lists no line. */
 int emtd_summator_edmn(const char* workdir, int with_ddv, int profile, int place);
 
+/* ---------------------------------------------------------------- time mesh (host only)
+ * Replace TimeDirectSolver (M/TimeDirectSolver/main.cpp:98-219, TimeStep::GenRegMesh and
+ * TimeStep::WriteTimes at M/TimeDirectSolver/TimeStep.cpp:76-171): the times of infite.0 in
+ * workdir become a grid of piecewise constant steps growing by the factor in tcff, written to
+ * infite.0; currentfunction and deltafunction are resampled on it (linear inside, end values
+ * outside). The inputs are kept as !infite.0, !currentfunction, !deltafunction. Every input is
+ * read before any file is renamed or written. */
+int emtd_time_direct_solver(const char* workdir);
+/* Replaces CorrectTimeMesh (M/CorrectTimeMesh/main.cpp:87-266): a decade of infite.0 whose
+ * length times tcff does not exceed the previous decade's length continues the previous
+ * decade's mean step instead; deltafunction is resampled on the new grid and written to
+ * deltafunction and currentfunction. The inputs are kept as !!infite.0, !!currentfunction,
+ * !!deltafunction. */
+int emtd_correct_time_mesh(const char* workdir);
+
 /* ---------------------------------------------------------------- request loop
  * No counterpart in the reference (which runs one model per process). For callers that serve
  * many forward-model requests on one GPU: each request is a calculation directory with the

@@ -9,7 +9,9 @@ Modules:
 
 * ``refio``  — readers of the reference's input files and writers of its output tables;
 * ``fem``    — element matrices, portrait, assembly, Dirichlet conditions;
-* ``calc``   — decades, the time schemes, sources, receivers and time differentiation.
+* ``calc``   — decades, the time schemes, sources, receivers and time differentiation;
+* ``summators`` — the output chain after the module (SumNonstat3DOnly, SummatorEDMN);
+* ``timemesh`` — the time-mesh steps before it (TimeDirectSolver, CorrectTimeMesh).
 
 Parity pinning (what checks each function beyond re-typing it):
 
