@@ -1152,11 +1152,12 @@ __global__ void __launch_bounds__(256) k_bwd_l21t(const DevFront* __restrict__ F
     for (; r + 64 < re; r += 128) {
         const double xa = v[r], xb = v[r + 64];
         double la[16], lb[16];
+        // non-temporal reads: the factor is read once per sweep
         for (int j = 0; j < 16; ++j) {
             const int c = wave + 4 * j;
             const T* col = Lf + int64_t(c0 + c) * m + r;
-            la[j] = c < nbc ? double(col[0]) : 0.0;
-            lb[j] = c < nbc ? double(col[64]) : 0.0;
+            la[j] = c < nbc ? double(__builtin_nontemporal_load(col)) : 0.0;
+            lb[j] = c < nbc ? double(__builtin_nontemporal_load(col + 64)) : 0.0;
         }
         for (int j = 0; j < 16; ++j) acc[j] += la[j] * xa;
         for (int j = 0; j < 16; ++j) acc[j] += lb[j] * xb;
