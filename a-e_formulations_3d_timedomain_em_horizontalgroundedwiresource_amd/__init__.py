@@ -89,6 +89,7 @@ EXPORTS = [
     "emtd_server_create", "emtd_server_submit", "emtd_server_wait", "emtd_server_poll", "emtd_server_drain",
     "emtd_server_get_stats", "emtd_server_destroy",
     "emtd_sum_nonstat_3d_only", "emtd_summator_edmn", "emtd_time_direct_solver", "emtd_correct_time_mesh",
+    "emtd_unload_anomal_nonstat",
 ]
 
 

@@ -2,7 +2,8 @@
 
 Usage: ``python -m <package>.build`` or ``build()`` from Python. Compiles every source under
 ``csrc/`` with hipcc (objects under ``_build/``), links ``libemtd.so`` next to this file, and
-builds the drop-in executables ``calcnonstat3dline``, ``sumnonstat3donly`` and ``summatoredmn``.
+builds the drop-in executables ``calcnonstat3dline``, ``sumnonstat3donly``, ``summatoredmn``,
+``timedirectsolver``, ``correcttimemesh`` and ``unloadanomalnonstat``.
 """
 from __future__ import annotations
 
@@ -16,9 +17,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemtd.so")
 EXE = os.path.join(HERE, "calcnonstat3dline")
-# drop-in executables: CalcNonstat3DLine, SumNonstat3DOnly, SummatorEDMN, TimeDirectSolver, CorrectTimeMesh
+# drop-in executables: CalcNonstat3DLine, SumNonstat3DOnly, SummatorEDMN, TimeDirectSolver, CorrectTimeMesh,
+# UnloadAnomalNonstat
 MAINS = {"calc_main.cpp": "calcnonstat3dline", "sum_main.cpp": "sumnonstat3donly", "edmn_main.cpp": "summatoredmn",
-         "timemesh_main.cpp": "timedirectsolver", "correct_main.cpp": "correcttimemesh"}
+         "timemesh_main.cpp": "timedirectsolver", "correct_main.cpp": "correcttimemesh",
+         "anomal_main.cpp": "unloadanomalnonstat"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -89,7 +89,7 @@ std::vector<double> read_points(const std::string& path) {
 
 }  // namespace
 
-ModelFiles read_model_files(const std::string& dir, int formulation) {
+ModelFiles read_mesh_files(const std::string& dir) {
     ModelFiles M;
     auto P = [&](const char* name) { return join_path(dir, name); };
     {  // inftry.dat (A2/in_out.cpp:474-508)
@@ -132,8 +132,18 @@ ModelFiles read_model_files(const std::string& dir, int formulation) {
         std::vector<double> mur, mur0;
         read_material_table(P("mu3d"), mur, mur0, M.nmat);
         M.mu.resize(M.nmat);
-        for (int i = 0; i < M.nmat; ++i) M.mu[i] = mur[i] * kMu0;
+        M.mu0.resize(M.nmat);
+        for (int i = 0; i < M.nmat; ++i) {
+            M.mu[i] = mur[i] * kMu0;
+            M.mu0[i] = mur0[i] * kMu0;
+        }
     }
+    return M;
+}
+
+ModelFiles read_model_files(const std::string& dir, int formulation) {
+    ModelFiles M = read_mesh_files(dir);
+    auto P = [&](const char* name) { return join_path(dir, name); };
     M.times = read_time_grid(P("infite.0"));
     M.recB = read_points(P("xyzVectorB"));
     M.recE = read_points(P("xyzVectorE"));

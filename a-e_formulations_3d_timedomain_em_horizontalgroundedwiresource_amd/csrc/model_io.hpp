@@ -15,6 +15,7 @@ struct ModelFiles {
     std::vector<int> edges;                  // n*2 (0-based)
     std::vector<int> ed;                     // kpar*12 (0-based)
     std::vector<double> sigma, sigma0, mu;   // per material (mu absolute)
+    std::vector<double> mu0;                 // per material, of the normal model (absolute)
     std::vector<double> times;
     std::vector<double> recB, recE;          // points*3
     int npls = 0;
@@ -43,6 +44,11 @@ void load_normal_field(const ModelFiles& M, int tnum, float* out);
 // Reads every input of the module from directory dir (A2/vec_prep_data.cpp:83-257, 834-853,
 // 982-1045; A2/time_approx.cpp:243-287; A2/TaskSP.cpp:467-535; A2/give_out_vec_loop.cpp:46-73).
 ModelFiles read_model_files(const std::string& dir, int formulation);
+
+// The mesh, edge numbering and material tables alone (inftry.dat, xyz.dat, nver.dat,
+// nvkat.dat, l13d.dat, tsize3d_.dat, nodesforedges.dat, edges.dat, sig3d, mu3d): the part of
+// read_model_files that the steps before the module need.
+ModelFiles read_mesh_files(const std::string& dir);
 
 // The time grid of infite.0 (A2/vec_prep_data.cpp:834-853).
 std::vector<double> read_time_grid(const std::string& path);

@@ -192,6 +192,15 @@ int emtd_time_direct_solver(const char* workdir);
  * !!deltafunction. */
 int emtd_correct_time_mesh(const char* workdir);
 
+/* ---------------------------------------------------------------- anomalous edges (host only)
+ * Replaces UnloadAnomalNonstat (M/UnloadAnomalNonstat/TaskSP.cpp:104-127,
+ * M/UnloadAnomalNonstat/time_approx.cpp:118-363): flags the edges of the elements whose
+ * conductivity differs from the normal model's, except those with a node above the last depth
+ * of z_sig_2d plus 1e-3, and writes isElemAnomal, mtrsizeforE0 (read by the calculation in
+ * primary/secondary mode), xyzVectorE0 and TgCompE0 (midpoints and unit tangents of the flagged
+ * edges); with a permeability anomaly also isElemAnomal_A, mtrsizeforA0, xyzVectorA0, TgCompA0. */
+int emtd_unload_anomal_nonstat(const char* workdir);
+
 /* ---------------------------------------------------------------- request loop
  * No counterpart in the reference (which runs one model per process). For callers that serve
  * many forward-model requests on one GPU: each request is a calculation directory with the

@@ -11,7 +11,8 @@ Modules:
 * ``fem``    — element matrices, portrait, assembly, Dirichlet conditions;
 * ``calc``   — decades, the time schemes, sources, receivers and time differentiation;
 * ``summators`` — the output chain after the module (SumNonstat3DOnly, SummatorEDMN);
-* ``timemesh`` — the time-mesh steps before it (TimeDirectSolver, CorrectTimeMesh).
+* ``timemesh`` — the time-mesh steps before it (TimeDirectSolver, CorrectTimeMesh);
+* ``anomal`` — the anomalous-edge export before a primary/secondary run (UnloadAnomalNonstat).
 
 Parity pinning (what checks each function beyond re-typing it):
 
