@@ -336,6 +336,26 @@ def test_calc_lookahead_factorization_is_exact(emtd, tmp_path, monkeypatch, form
         assert np.abs(got - want).max() / np.abs(want).max() < TOL64
 
 
+def test_calc_launch_options_are_exact(emtd, tmp_path, monkeypatch):
+    """The XCD-grouped update tile order and a CU mask on the look-ahead factorization's
+    streams change only where the work runs: receiver values are bit-identical to the
+    default launch."""
+    d, spec = make_model(tmp_path, cfg=1, cells=(12, 10, 8), steps=(4, 4))
+    vals = {}
+    for name, env in (("default", {}), ("options", {"EMTD_UPDATE_ORDER": "1", "EMTD_FACTOR_CU_RESERVE": "16"})):
+        for k in ("EMTD_UPDATE_ORDER", "EMTD_FACTOR_CU_RESERVE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        print("launch", name, flush=True)
+        c = emtd.Calc(d, 0, 2, emtd.FP64)
+        c.run()
+        vals[name] = (c.receiver_values(0), c.receiver_values(1))
+        c.close()
+    for k in range(2):
+        np.testing.assert_array_equal(vals["options"][k], vals["default"][k])
+
+
 def test_output_chain_executables_match_oracle(emtd, mg, tmp_path):
     """Module -> SumNonstat3DOnly -> (the orchestrator's rename exall.N -> exall<profile>_<place>)
     -> SummatorEDMN through the three drop-in executables, against the same chain on the
