@@ -650,9 +650,11 @@ void CalcCore::refine(const double* b, double* x, const double* predictor) {
         launch_axpy_rows(stream_, nn_, n_, npls_, ir_d_.p, x);
         ++refine_iterations_;
     }
-    // fallback: FP64 factor for the rest of this decade
+    // fallback: FP64 factor for the rest of this decade; as after an FP32 pivot failure, the
+    // decades with a smaller or equal mult (weaker mass term) go to FP64 directly
     ++fallbacks_;
     use64_ = true;
+    fp32_failed_mult_ = std::max(fp32_failed_mult_, cur_mult_);
     if (!c64_) c64_ = std::make_unique<GpuCholesky<double>>(*S_, stream_);
     c64_->factorize_system(dest_off_.p, dest_diag_.p, nnz_, n_, b_ggl_.p, b_di_.p, c_ggl_.p, c_di_.p, cur_mult_);
     if (c64_->check_info() >= 0) throw Error(EMTD_ERR_NOTSPD, "FP64 fallback factorization failed");
