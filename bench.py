@@ -293,8 +293,8 @@ def main():
         log(f"[bench] copied {args.copy_calibration} bytes 3 times")
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              f"the multi-GPU design is described in DESIGN.md and not measured.", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU (asked for --gpus {args.gpus}); "
+              f"sharding over several GPUs is not implemented, so there is nothing to measure.", file=sys.stderr)
         sys.exit(2)
     emtd = importlib.import_module(PKG)
     if args.config in (4, 5):
