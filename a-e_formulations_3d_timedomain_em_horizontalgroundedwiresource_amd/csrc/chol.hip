@@ -8,7 +8,7 @@
 //   extend-add child update U_c -> parent front          (one wavefront per update column)
 //   panel      diagonal NB x NB block (factored in LDS by every workgroup) and the panel
 //              rows below it                             (one wavefront per 64 rows)
-//   syrk       update inside the 128-column outer block  (64x64 tiles, MFMA 16x16x4)
+//   syrk       update inside the outer block (256 columns by default; 64x64 or 128x128 tiles)
 //   update     trailing pivot columns per outer block, and the update block U once per
 //              front with K = p                          (128x128 tiles, MFMA 16x16x4)
 //

@@ -36,7 +36,7 @@ struct DevFront {
 // Launch parameters of the factorization and of the triangular sweeps. Every choice gives the
 // same factor up to rounding; the best one depends on the front sizes (see tune.hpp).
 struct Tuning {
-    int outer_block = 128;          // columns per outer block (a multiple of the 32-column panel)
+    int outer_block = 256;          // columns per outer block (a multiple of the 32-column panel)
     int wide_tile_cols = 256;       // update targets at least this wide use 128x128 tiles
     int lookahead_lds_pad = 0;      // extra LDS bytes of the look-ahead bulk update
     int split_level_fronts = 1024;  // levels with fewer fronts use the split sweeps
